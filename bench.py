@@ -1,0 +1,201 @@
+"""Benchmark: path-tracing throughput on one MI355X.
+
+A step renders the mesh configuration once: a seeded displaced icosphere
+(69,620 triangles) in the Cornell enclosure with one sphere light, 512x512,
+64 samples per pixel, depth 8, next-event estimation, then 5 a-trous
+passes of the edge-aware denoise.  Scene flattening and the GPU BVH build
+happen once before the timed region; inputs are resident in HBM.
+
+Prints one JSON line:
+  value        Mrays/s over the timed steps (rays counted as the reference
+               counts them: one per bounce-loop iteration plus one per
+               shadow ray cast)
+  roofline     the closest-hit (extend) kernel: algorithmic bytes
+               64*node pops + 36*triangle tests + 16*primitive tests per
+               launch over its event-timed average duration, against 8 TB/s
+  cpu_baseline the CPU oracle (fp32 restatement of the reference kernel)
+               on a bounded pixel subset of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+NODE_B, TRI_B, PRIM_B = 64, 36, 16
+
+
+def _load_traffic(name):
+    """HBM bytes per extend launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel of known size), if recorded."""
+    p = ROOT / "profiles" / f"traffic_{name}.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def cpu_baseline(name, target_seconds=15.0, threads=None):
+    """Oracle fp32 render of a pixel subset of the same workload."""
+    from oracle import oracle as orc
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.extract import flatten_scene
+    from tests import helpers
+
+    world, W, H, st = scenes.CONFIGS[name]()
+    flat = flatten_scene(world)
+    osc = helpers.flat_to_oracle(flat)
+    cam = world.active_camera
+    ocam = helpers.oracle_camera(cam, W, H)
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    prm = orc.params(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                     world._background_color, 0)
+    threads = threads or os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    n_pix = 256
+    spp = st["samples"]
+    elapsed, rays, pix_done = 0.0, 0, 0
+    while elapsed < target_seconds and pix_done < W * H:
+        pix = rng.choice(W * H, size=n_pix, replace=False).astype(np.int32)
+        acc = orc.Accum(n_pix)
+        t0 = time.perf_counter()
+        orc.render(osc, ocam, prm, 0, spp, acc, threads=threads, pixels=pix)
+        elapsed += time.perf_counter() - t0
+        rays += acc.counts.rays
+        pix_done += n_pix
+        n_pix = min(n_pix * 2, 65536)
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{pix_done} random pixels x {spp} spp of the {W}x{H} {name} frame, fp32 oracle, "
+                      f"{elapsed:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="mesh", choices=["cornell", "mesh", "large"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--samples-per-pass", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(pixels/samples would shard across GPUs with an accumulator all-reduce)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.denoise import edge_aware_denoise, linear_to_display
+    from roxy_dcc_amd.extract import extract_scene
+
+    nat.require_gpu()
+    world, W, H, st = scenes.CONFIGS[args.config]()
+    t0 = time.perf_counter()
+    ex = extract_scene(world)
+    torch.cuda.synchronize()
+    t_extract = time.perf_counter() - t0
+    sc = ex.scene
+    spp = st["samples"]
+    per_pass = args.samples_per_pass or R.default_samples_per_pass(W, H, spp)
+    integ = R.Integrator(sc, W, H, per_pass)
+    frame = R.DeviceFrame(W, H)
+    cam = R.camera_struct(world.active_camera, W, H)
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    denoise_passes = st.get("denoise_passes", 0)
+
+    def step(flags):
+        prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                              world._background_color, 0, per_pass, flags)
+        done = 0
+        while done < spp:
+            n = min(per_pass, spp - done)
+            integ.render(frame, cam, prm, done, n)
+            done += n
+        if denoise_passes:
+            x = frame.color
+            for k in range(denoise_passes):
+                x = edge_aware_denoise(x, 1, 0.08, 0.8, normal=frame.normal, albedo=frame.albedo,
+                                       depth=frame.depth, dilation=1 << k)
+            linear_to_display(x)
+
+    for _ in range(args.warmup):
+        step(0)
+    # traversal work of one step (deterministic: same counts every step)
+    integ.reset_counts()
+    step(nat.RX_FLAG_COUNT_TRAFFIC)
+    torch.cuda.synchronize()
+    work = integ.counts()
+    integ.reset_counts()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(nat.RX_FLAG_PROFILE)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    counts = integ.counts()
+    prof = integ.profile
+    rays = counts["rays"]
+    ms_step = dt * 1e3 / args.steps
+
+    n_ext = max(prof.n_extend, 1)
+    alg_bytes_step = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"] + PRIM_B * work["prim_tests"]
+    ext_launches_step = prof.n_extend / args.steps
+    bytes_per_launch = alg_bytes_step / max(ext_launches_step, 1)
+    ms_per_launch = prof.ms_extend / n_ext
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    traffic = _load_traffic(args.config)
+
+    out = {
+        "metric": "Mrays/s",
+        "value": rays / dt / 1e6,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {W}x{H}, {spp} spp, depth {st['max_depth']}, NEE one, "
+                               f"{sc.n_tris} triangles, {sc.n_prims} primitives, {denoise_passes} a-trous denoise "
+                               "passes",
+                   "samples_per_pass": per_pass, "parallelism": "single GPU"},
+        "renders_per_s": args.steps / dt,
+        "samples_per_s": W * H * spp * args.steps / dt,
+        "rays_per_step": rays // args.steps,
+        "stage_ms_per_step": {k: getattr(prof, "ms_" + k) / args.steps for k in
+                              ("generate", "extend", "shade", "connect", "accumulate")},
+        "scene_setup_s": t_extract,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_extend (closest-hit traversal)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms_per_launch,
+                     "launches_per_step": ext_launches_step},
+        "traversal_work_per_step": work,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

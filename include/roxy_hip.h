@@ -1,0 +1,320 @@
+/*
+ * roxy_hip.h — C ABI of the MI355X (gfx950) path-tracing backend.
+ *
+ * Every entry point returns an int status (RX_OK == 0) and never throws
+ * across the ABI; rx_last_error() returns a thread-local message for the
+ * last failing call on the calling thread.  All pointers named `d_*` or
+ * typed as device structures are HIP device pointers that the caller owns
+ * (the library borrows them for the duration of the call).  `stream` is a
+ * hipStream_t passed as void*; NULL means the null stream.  Work is
+ * stream-ordered: a call returns once its kernels are enqueued unless it
+ * says otherwise.
+ *
+ * Reference interface each entry point replaces (GuyontheCouch006/roxy-dcc):
+ *   rx_bvh_*         rendering/taichi/bvh_builder.py:315-558 (GPUBVHBuilder.build,
+ *                    _morton_order, _build_morton_node) and :560-655 (upload)
+ *   rx_closest_hit   rendering/taichi/intersect.py:249-309 (scene_intersect,
+ *                    bvh_intersect :81-152)
+ *   rx_any_hit       rendering/taichi/intersect.py:222-246 (scene_occluded,
+ *                    bvh_occluded :155-219)
+ *   rx_render        rendering/taichi/trace.py:239-283 (render_kernel) driven by
+ *                    rendering/taichi_renderer.py:140-156 (one launch per sample)
+ *   rx_denoise       rendering/denoise.py:23-104 (edge_aware_denoise)
+ *   rx_display       rendering/denoise.py:107-108 (linear_to_display)
+ */
+#ifndef ROXY_HIP_H
+#define ROXY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RX_ABI_VERSION 1
+
+/* status codes */
+#define RX_OK 0
+#define RX_ERR_INVALID 1     /* bad argument (maps to ValueError) */
+#define RX_ERR_CAPACITY 2    /* capacity exceeded (maps to AssertionError) */
+#define RX_ERR_HIP 3         /* a HIP runtime call failed */
+#define RX_ERR_WORKSPACE 4   /* workspace too small */
+
+/* Primitive kinds written by rx_closest_hit (`kind` output). */
+#define RX_KIND_MISS 0
+#define RX_KIND_PRIM 1   /* analytic primitive slot (id = slot index) */
+#define RX_KIND_BVH 2    /* BVH triangle (id = index in BVH leaf order) */
+
+/* Analytic primitive types: rendering/taichi/intersect.py:265-282 */
+#define RX_PRIM_SPHERE 0
+#define RX_PRIM_PLANE 1
+#define RX_PRIM_CUBE 2
+#define RX_PRIM_TRIANGLE 4
+
+/* Material types: scene/materials.py:60,88,117,163,183 */
+#define RX_MAT_DIFFUSE 0
+#define RX_MAT_METAL 1
+#define RX_MAT_DIELECTRIC 2
+#define RX_MAT_EMISSIVE 3
+#define RX_MAT_GLOSSY 4
+
+/* Largest leaf the packed node link can describe, and largest triangle
+ * count (the packed link keeps 23 bits of tri_start). */
+#define RX_MAX_LEAF_TRIS 255
+#define RX_MAX_TRIANGLES (1 << 23)
+/* Deepest BVH (root depth 0) the traversal stacks support. */
+#define RX_MAX_BVH_DEPTH 63
+
+/*
+ * BVH2 node, 64 bytes, one per reference node in the reference's preorder
+ * numbering (node i here is node i of GPUBVHBuilder.nodes).  Each record
+ * holds the boxes of BOTH children, so one 64-byte fetch answers both
+ * child slab tests of rendering/taichi/intersect.py:121-126.
+ *   box[0..5]  left child  min xyz, max xyz
+ *   box[6..11] right child min xyz, max xyz
+ *   link[c]    child c: its node index if it is interior, otherwise
+ *              ~((tri_start << 8) | tri_count) (negative) for a leaf
+ *   node[c]    child c's node index (interior or leaf), for export
+ * A leaf's own record has box[0..5] = its own box, link = {~leaf, -1},
+ * node = {-1, -1}.
+ */
+typedef struct rx_node {
+    float box[12];
+    int32_t link[2];
+    int32_t node[2];
+} rx_node;
+
+/* Root record of a BVH: the root's own box (tested first, as
+ * intersect.py:95-100 does) and its link (0, or ~leaf if the root is a
+ * leaf).  depth = deepest node depth (root = 0). */
+typedef struct rx_bvh_header {
+    float bmin[3];
+    int32_t root_link;
+    float bmax[3];
+    int32_t depth;
+} rx_bvh_header;
+
+/* Triangle geometry in BVH leaf order, 48 bytes: v0, e1 = v1 - v0,
+ * e2 = v2 - v0 (the first two lines of intersect.py:57-78, done once). */
+typedef struct rx_tri {
+    float v0[3];
+    float e1[3];
+    float e2[3];
+    float pad[3];
+} rx_tri;
+
+/* Shading data in BVH leaf order, 48 bytes: per-vertex normals
+ * (fields.py:62-64), palette material index (fields.py:71), has_uv. */
+typedef struct rx_tri_shade {
+    float n0[3];
+    float n1[3];
+    float n2[3];
+    int32_t mat;
+    int32_t has_uv;
+    int32_t pad;
+} rx_tri_shade;
+
+/* Analytic primitive slot, 48 bytes (fields.py:22-33).
+ *   sphere:   a = center xyz, radius
+ *   plane:    a = normal xyz, offset   (n.p + offset = 0)
+ *   cube:     a = center xyz, 0; b = half extents xyz, 0
+ *   triangle: v0 = a.xyz, v1 = (a.w, b.x, b.y), v2 = (b.z, b.w, c.x) */
+typedef struct rx_prim {
+    float a[4];
+    float b[4];
+    float c[3];
+    int32_t type;
+} rx_prim;
+
+/* Material record, 32 bytes (fields.py:35-40 and :73-79). */
+typedef struct rx_material {
+    int32_t type;
+    float albedo[3];
+    float roughness;
+    float ior;
+    float emission;
+    int32_t texture;
+} rx_material;
+
+/* Emissive-sphere light, 32 bytes (fields.py:43-47). */
+typedef struct rx_light {
+    float center[3];
+    float radius;
+    float albedo[3];
+    float emission;
+} rx_light;
+
+/* Scene view: device pointers plus host-side sizes. */
+typedef struct rx_scene {
+    const rx_node* nodes;
+    const rx_bvh_header* header;
+    const rx_tri* tris;
+    const rx_tri_shade* tri_shade;
+    const rx_material* materials;
+    const rx_prim* prims;
+    const rx_material* prim_materials;
+    const rx_light* lights;
+    int32_t n_nodes;
+    int32_t n_tris;
+    int32_t n_materials;
+    int32_t n_prims;
+    int32_t n_lights;
+    int32_t bvh_depth;
+} rx_scene;
+
+/* Pinhole camera (rendering/taichi/camera.py:23-30). */
+typedef struct rx_camera {
+    float pos[3];
+    float fwd[3];
+    float right[3];
+    float up[3];
+    float fov_tan;
+    float aspect;
+    int32_t width;
+    int32_t height;
+} rx_camera;
+
+/* Flags for rx_render_params.flags */
+#define RX_FLAG_PROFILE 1        /* time each stage with HIP events (synchronises) */
+#define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
+
+/* Integrator settings (render_kernel arguments, trace.py:239-254). */
+typedef struct rx_render_params {
+    int32_t max_depth;
+    int32_t use_sky;
+    int32_t direct_light_mode;       /* 0 = one (random light), 1 = all */
+    int32_t direct_light_max_depth;
+    float sample_clamp;              /* <= 0 disables the clamp */
+    float bg[3];
+    uint32_t seed_lo;
+    uint32_t seed_hi;
+    int32_t flags;
+    int32_t samples_per_pass;        /* samples traced together per wavefront pass */
+} rx_render_params;
+
+/* Per-pixel running means written by rx_render (fields.py:14-19). */
+typedef struct rx_accum {
+    float* color;    /* H*W*3 linear radiance */
+    float* normal;   /* H*W*3 first-hit normal */
+    float* albedo;   /* H*W*3 first-hit albedo */
+    float* depth;    /* H*W   first-hit t */
+    float* display;  /* H*W*3 sqrt(min(color, 1)) */
+} rx_accum;
+
+/* Device-side counters (64-bit).  rays = extension rays + shadow rays cast,
+ * counted as trace.py:78-79,150-151 does. */
+typedef struct rx_counters {
+    unsigned long long rays;
+    unsigned long long shadow_rays;
+    unsigned long long node_pops;      /* closest-hit traversal only */
+    unsigned long long tri_tests;      /* closest-hit traversal only */
+    unsigned long long prim_tests;     /* closest-hit traversal only */
+    unsigned long long shadow_node_pops;
+    unsigned long long shadow_tri_tests;
+    unsigned long long shadow_prim_tests;
+} rx_counters;
+
+/* Host-side stage timings filled when RX_FLAG_PROFILE is set. */
+typedef struct rx_profile {
+    float ms_generate;
+    float ms_extend;
+    float ms_shade;
+    float ms_connect;
+    float ms_accumulate;
+    int32_t n_extend;
+    int32_t n_connect;
+    int32_t n_passes;
+} rx_profile;
+
+/* ---- library ---------------------------------------------------------- */
+int rx_version(void);
+const char* rx_last_error(void);
+
+/* ---- BVH build: rendering/taichi/bvh_builder.py:412-486 ----------------
+ * Node count and depth of the count-median Morton tree over n_tris
+ * triangles with leaves of at most max_leaf (closed form: the topology of
+ * _build_morton_node depends only on the count). */
+int rx_bvh_morton_shape(int n_tris, int max_leaf, int* n_nodes, int* depth);
+int rx_bvh_workspace_size(int n_tris, int max_leaf, size_t* bytes);
+
+/* Reference-layout node arrays (fields.py:50-56); every pointer is device. */
+typedef struct rx_bvh_ref {
+    float* aabb_min;    /* n_nodes*3 */
+    float* aabb_max;    /* n_nodes*3 */
+    int32_t* left;      /* n_nodes, -1 at leaves */
+    int32_t* right;     /* n_nodes, -1 at leaves */
+    int32_t* tri_start; /* n_nodes, -1 at interior nodes */
+    int32_t* tri_count; /* n_nodes, 0 at interior nodes */
+} rx_bvh_ref;
+
+/* GPU Morton build (bvh_builder.py:347-368, :416-486).  d_v0/d_v1/d_v2 are
+ * n_tris*3 floats.  Writes the reference-layout arrays into *ref (device
+ * arrays the caller sized with rx_bvh_morton_shape), the packed 64-byte
+ * nodes, the header and the leaf order (_finish_ordered_arrays'
+ * ordered_tris).  Synchronous on `stream` only. */
+int rx_bvh_build_morton(const float* d_v0, const float* d_v1, const float* d_v2,
+                        int n_tris, int max_leaf,
+                        void* d_workspace, size_t workspace_bytes,
+                        const rx_bvh_ref* ref, rx_node* d_nodes,
+                        rx_bvh_header* d_header, int32_t* d_order,
+                        void* stream);
+
+/* Pack reference-layout node arrays (built elsewhere, e.g. by the median
+ * builders) into the 64-byte nodes and the header.  `depth` is the tree's
+ * deepest node depth. */
+int rx_bvh_pack(const rx_bvh_ref* ref, int n_nodes, int depth,
+                rx_node* d_nodes, rx_bvh_header* d_header, void* stream);
+
+/* Gather triangles into leaf order (bvh_builder.py:527-544) and precompute
+ * edges.  d_n* may be NULL (face normal is used); d_mat_idx may be NULL
+ * (material 0); d_has_uv may be NULL. */
+int rx_bvh_pack_triangles(const int32_t* d_order, int n_tris,
+                          const float* d_v0, const float* d_v1, const float* d_v2,
+                          const float* d_n0, const float* d_n1, const float* d_n2,
+                          const int32_t* d_mat_idx, const int32_t* d_has_uv,
+                          rx_tri* d_tris, rx_tri_shade* d_shade, void* stream);
+
+/* ---- queries: rendering/taichi/intersect.py ------------------------------
+ * d_rays: 6*n floats, SoA (ox[n], oy[n], oz[n], dx[n], dy[n], dz[n]).
+ * Outputs are n-element device arrays; any output pointer may be NULL. */
+int rx_closest_hit(const rx_scene* scene, const float* d_rays, int n,
+                   float* d_t, int32_t* d_id, int32_t* d_kind,
+                   float* d_u, float* d_v, rx_counters* d_counters, void* stream);
+
+int rx_any_hit(const rx_scene* scene, const float* d_rays, const float* d_tmax,
+               int n, int32_t* d_occluded, rx_counters* d_counters, void* stream);
+
+/* ---- integrator: rendering/taichi/trace.py:139-283 ----------------------
+ * Wavefront path tracer: generate, extend, shade, connect, accumulate.
+ * Renders samples [first_sample, first_sample + n_samples) and folds them
+ * into *accum in sample order with trace.py's running mean.  The
+ * workspace must hold rx_render_workspace_size() bytes for the same
+ * width, height and params->samples_per_pass.  *profile is written (and
+ * the call synchronises) only when RX_FLAG_PROFILE is set. */
+int rx_render_workspace_size(int width, int height, int samples_per_pass,
+                             int n_lights, size_t* bytes);
+int rx_render(const rx_scene* scene, const rx_camera* camera,
+              const rx_render_params* params, int first_sample, int n_samples,
+              void* d_workspace, size_t workspace_bytes,
+              const rx_accum* accum, rx_counters* d_counters,
+              rx_profile* profile, void* stream);
+
+/* ---- denoise: rendering/denoise.py:23-108 -------------------------------
+ * Guided bilateral filter over a (2*radius+1)^2 window whose taps are
+ * `dilation` pixels apart (dilation 1 is the reference filter), reflect
+ * padding.  d_normal/d_albedo/d_depth may be NULL (guide skipped).  d_out
+ * must not alias d_color. */
+int rx_denoise(const float* d_color, const float* d_normal, const float* d_albedo,
+               const float* d_depth, int width, int height, int radius, int dilation,
+               float sigma_color, float sigma_normal, float sigma_albedo,
+               float sigma_depth, float amount, float* d_out, void* stream);
+
+/* sqrt(min(max(x, 0), 1)) elementwise over n floats. */
+int rx_display(const float* d_linear, int n, float* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROXY_HIP_H */

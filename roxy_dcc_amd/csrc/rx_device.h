@@ -1,0 +1,490 @@
+// rx_device.h — device math, sampling and BVH traversal for gfx950.
+//
+// Floating-point contract: every expression here is evaluated in IEEE fp32
+// (fp64 for the jitter hash) with no FMA contraction (the library is
+// compiled with -ffp-contract=off and the pragma below), correctly rounded
+// division and sqrt, and the operand order the reference's Taichi code
+// spells out (dot = (x*x' + y*y') + z*z', left-to-right products).  That
+// is what makes closest-hit ids and t bit-identical to a CPU restatement
+// that follows the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "roxy_hip.h"
+
+#pragma clang fp contract(off)
+
+namespace rx {
+
+// ---------------------------------------------------------------- vectors
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// v * (1 / |v|): one correctly rounded divide, then three products.
+__device__ __forceinline__ f3 normalize(f3 v) {
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return v * inv;
+}
+__device__ __forceinline__ float fract(float x) { return x - floorf(x); }
+__device__ __forceinline__ f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Counter-based generator of Salmon, Moraes, Dror and Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3" (SC'11): ten rounds of two 32x32->64
+// multiplies with Weyl-sequence key bumps.
+struct u4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = M0 * c.x, hi0 = __umulhi(M0, c.x);
+        uint32_t lo1 = M1 * c.z, hi1 = __umulhi(M1, c.z);
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Top 24 bits -> [0, 1), exactly representable.
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------- sin/cos
+// sin and cos of 2*pi*u for u in [0, 1): quadrant split on 4u (exact),
+// fold to [0, pi/4] and Taylor polynomials whose truncation error is far
+// below half an ulp there.  Plain fp32 arithmetic, so a CPU build of the
+// same steps gives the same bits.
+__device__ __forceinline__ void sincos_2pi(float u, float* s_out, float* c_out) {
+    float u4x = u * 4.0f;
+    float q = floorf(u4x);
+    float f = u4x - q;  // [0, 1) exact
+    bool swap = f > 0.5f;
+    float g = swap ? (1.0f - f) : f;           // [0, 0.5] exact
+    float x = g * 1.57079637050628662109375f;  // pi/2 rounded to fp32
+    float x2 = x * x;
+    float s = x + x * x2 * (-1.66666666666666666667e-1f +
+                            x2 * (8.33333333333333333333e-3f +
+                                  x2 * (-1.98412698412698412698e-4f + x2 * 2.75573192239858906526e-6f)));
+    float c = 1.0f + x2 * (-0.5f + x2 * (4.16666666666666666667e-2f +
+                                         x2 * (-1.38888888888888888889e-3f +
+                                               x2 * (2.48015873015873015873e-5f +
+                                                     x2 * -2.75573192239858906526e-7f))));
+    if (swap) {
+        float t = s;
+        s = c;
+        c = t;
+    }
+    int qi = (int)q & 3;
+    float so, co;
+    if (qi == 0) {
+        so = s;
+        co = c;
+    } else if (qi == 1) {
+        so = c;
+        co = -s;
+    } else if (qi == 2) {
+        so = -s;
+        co = -c;
+    } else {
+        so = -c;
+        co = s;
+    }
+    *s_out = so;
+    *c_out = co;
+}
+
+// fp64 sine for the camera jitter hash: three-piece Cody-Waite reduction
+// by pi/2 (30-bit pieces, exact products for |k| < 2^23) and Taylor series
+// on [-pi/4, pi/4].
+__device__ __forceinline__ double sin_hash_d(double x) {
+    const double P1 = 1.570796325802803;       // pi/2, first 30 bits
+    const double P2 = 9.920935791635221e-10;   // next 30 bits
+    const double P3 = 5.170182981794105e-19;   // remainder
+    double k = rint(x * 6.36619772367581382433e-01);
+    double r = ((x - k * P1) - k * P2) - k * P3;
+    double r2 = r * r;
+    double s = r + r * r2 * (-1.0 / 6.0 + r2 * (1.0 / 120.0 + r2 * (-1.0 / 5040.0 + r2 * (1.0 / 362880.0 +
+                 r2 * (-1.0 / 39916800.0 + r2 * (1.0 / 6227020800.0 + r2 * (-1.0 / 1307674368000.0 +
+                 r2 * (1.0 / 355687428096000.0))))))));
+    double c = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24.0 + r2 * (-1.0 / 720.0 + r2 * (1.0 / 40320.0 +
+                 r2 * (-1.0 / 3628800.0 + r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0 +
+                 r2 * (1.0 / 20922789888000.0 + r2 * (-1.0 / 6402373705728000.0)))))))));
+    long long qi = ((long long)k) & 3;
+    if (qi == 0) return s;
+    if (qi == 1) return c;
+    if (qi == 2) return -s;
+    return -c;
+}
+
+// rendering/taichi/camera.py:9-20: fract(sin(127.1 x + 311.7 y + 74.7 s) * 43758.5453123)
+__device__ __forceinline__ float hash_float(int px, int py, float salt) {
+    float seed = ((float)px * 127.1f + (float)py * 311.7f) + salt * 74.7f;
+    float sv = (float)sin_hash_d((double)seed);
+    return fract(sv * 43758.5453123f);
+}
+
+// rendering/taichi/camera.py:23-30 with the R2 jitter of :16-20.
+__device__ __forceinline__ f3 camera_dir(const rx_camera& cam, int px, int py, int frame) {
+    float i = (float)(frame + 1);
+    float jx = fract(hash_float(px, py, 0.0f) + i * 0.7548776662466927f);
+    float jy = fract(hash_float(px, py, 1.0f) + i * 0.5698402909980532f);
+    float u = ((float)px + jx) / (float)cam.width * 2.0f - 1.0f;
+    float v = 1.0f - ((float)py + jy) / (float)cam.height * 2.0f;
+    float ndc_x = u * cam.aspect * cam.fov_tan;
+    float ndc_y = v * cam.fov_tan;
+    f3 d = (ld3(cam.fwd) + ld3(cam.right) * ndc_x) + ld3(cam.up) * ndc_y;
+    return normalize(d);
+}
+
+// ---------------------------------------------------------------- intersection
+// rendering/taichi/intersect.py:17-25
+__device__ __forceinline__ f3 inv_dir(f3 d) {
+    return mk(fabsf(d.x) > 1e-10f ? 1.0f / d.x : 1e30f, fabsf(d.y) > 1e-10f ? 1.0f / d.y : 1e30f,
+              fabsf(d.z) > 1e-10f ? 1.0f / d.z : 1e30f);
+}
+
+// rendering/taichi/intersect.py:28-43.  lo/hi point at xyz triples.
+__device__ __forceinline__ bool slab(f3 ro, f3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float limit, float* t_enter_out) {
+    float tx0 = (lx - ro.x) * inv.x, tx1 = (hx - ro.x) * inv.x;
+    float ty0 = (ly - ro.y) * inv.y, ty1 = (hy - ro.y) * inv.y;
+    float tz0 = (lz - ro.z) * inv.z, tz1 = (hz - ro.z) * inv.z;
+    float t_enter = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
+    float t_exit = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
+    *t_enter_out = t_enter;
+    return t_exit >= t_enter && t_exit >= 0.001f && t_enter <= limit;
+}
+
+// Moeller-Trumbore of rendering/taichi/intersect.py:57-78 on (v0, e1, e2).
+__device__ __forceinline__ float tri_hit(f3 ro, f3 rd, f3 v0, f3 e1, f3 e2, float* uo, float* vo) {
+    f3 h = cross(rd, e2);
+    float a = dot(e1, h);
+    float t = -1.0f, u = 0.0f, v = 0.0f;
+    if (fabsf(a) > 1e-8f) {
+        float f = 1.0f / a;
+        f3 s = ro - v0;
+        u = f * dot(s, h);
+        if (u >= 0.0f && u <= 1.0f) {
+            f3 q = cross(s, e1);
+            v = f * dot(rd, q);
+            if (v >= 0.0f && u + v <= 1.0f) {
+                float tc = f * dot(e2, q);
+                if (tc > 0.001f) t = tc;
+            }
+        }
+    }
+    *uo = u;
+    *vo = v;
+    return t;
+}
+
+// scene/primitives.py:90-105
+__device__ __forceinline__ float sphere_hit(f3 ro, f3 rd, f3 c, float r) {
+    f3 oc = ro - c;
+    float h = dot(rd, oc);
+    float cc = dot(oc, oc) - r * r;
+    float disc = h * h - cc;
+    float t = -1.0f;
+    if (disc >= 0.0f) {
+        float sq = sqrtf(disc);
+        t = -h - sq;
+        if (t < 0.001f) t = -h + sq;
+        if (t < 0.001f) t = -1.0f;
+    }
+    return t;
+}
+
+// scene/primitives.py:145-154
+__device__ __forceinline__ float plane_hit(f3 ro, f3 rd, f3 n, float off) {
+    float denom = dot(n, rd);
+    float t = -1.0f;
+    if (fabsf(denom) >= 1e-6f) {
+        float tc = -(dot(n, ro) + off) / denom;
+        if (tc >= 0.001f) t = tc;
+    }
+    return t;
+}
+
+// scene/primitives.py:187-201 (raw division: inf/NaN resolved by min/max)
+__device__ __forceinline__ float cube_hit(f3 ro, f3 rd, f3 c, f3 e) {
+    f3 lo = c - e, hi = c + e;
+    float tx0 = (lo.x - ro.x) / rd.x, tx1 = (hi.x - ro.x) / rd.x;
+    float ty0 = (lo.y - ro.y) / rd.y, ty1 = (hi.y - ro.y) / rd.y;
+    float tz0 = (lo.z - ro.z) / rd.z, tz1 = (hi.z - ro.z) / rd.z;
+    float t_enter = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
+    float t_exit = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
+    float t = -1.0f;
+    if (t_exit >= t_enter && t_exit >= 0.001f) t = t_enter >= 0.001f ? t_enter : t_exit;
+    return t;
+}
+
+// scene/primitives.py:203-215
+__device__ __forceinline__ f3 cube_normal(f3 p, f3 c, f3 e) {
+    f3 d = mk((p.x - c.x) / e.x, (p.y - c.y) / e.y, (p.z - c.z) / e.z);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    if (ax >= ay && ax >= az) return mk(d.x > 0.0f ? 1.0f : -1.0f, 0.0f, 0.0f);
+    if (ay >= az) return mk(0.0f, d.y > 0.0f ? 1.0f : -1.0f, 0.0f);
+    return mk(0.0f, 0.0f, d.z > 0.0f ? 1.0f : -1.0f);
+}
+
+__device__ __forceinline__ float prim_hit(const rx_prim& p, f3 ro, f3 rd) {
+    f3 a = mk(p.a[0], p.a[1], p.a[2]);
+    switch (p.type) {
+        case RX_PRIM_SPHERE:
+            return sphere_hit(ro, rd, a, p.a[3]);
+        case RX_PRIM_PLANE:
+            return plane_hit(ro, rd, a, p.a[3]);
+        case RX_PRIM_CUBE:
+            return cube_hit(ro, rd, a, mk(p.b[0], p.b[1], p.b[2]));
+        case RX_PRIM_TRIANGLE: {
+            f3 v1 = mk(p.a[3], p.b[0], p.b[1]), v2 = mk(p.b[2], p.b[3], p.c[0]);
+            float u, v;
+            return tri_hit(ro, rd, a, v1 - a, v2 - a, &u, &v);
+        }
+        default:
+            return -1.0f;
+    }
+}
+
+// Outward normal of primitive p at distance t (rendering/taichi/intersect.py:265-282).
+__device__ __forceinline__ f3 prim_normal(const rx_prim& p, f3 ro, f3 rd, float t) {
+    f3 a = mk(p.a[0], p.a[1], p.a[2]);
+    switch (p.type) {
+        case RX_PRIM_SPHERE:
+            return normalize((ro + rd * t) - a);
+        case RX_PRIM_PLANE:
+            return a;
+        case RX_PRIM_CUBE:
+            return cube_normal(ro + rd * t, a, mk(p.b[0], p.b[1], p.b[2]));
+        case RX_PRIM_TRIANGLE: {
+            f3 v1 = mk(p.a[3], p.b[0], p.b[1]), v2 = mk(p.b[2], p.b[3], p.c[0]);
+            return normalize(cross(v1 - a, v2 - a));
+        }
+        default:
+            return mk(0.0f, 0.0f, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------- traversal
+// Per-lane traversal counters (summed per wave by the caller).
+struct trav_count {
+    uint32_t pops, tris, prims;
+};
+
+// The LDS stack of one lane: entry k lives at base[k * STRIDE] as
+// (link, float bits of t_enter); STRIDE = threads per block keeps the 64
+// lanes of a wave on distinct banks whatever their stack depths.
+template <int STRIDE>
+struct lane_stack {
+    int2* base;
+    __device__ __forceinline__ void put(int k, int link, float t) { base[k * STRIDE] = make_int2(link, __float_as_int(t)); }
+    __device__ __forceinline__ int2 get(int k) const { return base[k * STRIDE]; }
+};
+
+// Closest hit over the BVH: rendering/taichi/intersect.py:81-152.  Visit
+// order, pruning and acceptance follow the reference exactly: pop, skip if
+// the stored entry t exceeds the current closest t, test a leaf's
+// triangles in ascending order with 0.001 < t < closest, or test both
+// child boxes against the current closest and push the far child first
+// (equal entry t: left pushed first, so the right child pops first).
+template <int STRIDE, bool COUNT>
+__device__ __forceinline__ void bvh_closest(const rx_node* __restrict__ nodes, const rx_tri* __restrict__ tris,
+                                            const rx_bvh_header& hdr, f3 ro, f3 rd, lane_stack<STRIDE> stk,
+                                            float& best_t, int& best_i, float& best_u, float& best_v,
+                                            trav_count& cnt) {
+    f3 inv = inv_dir(rd);
+    best_t = 1e9f;
+    best_i = -1;
+    best_u = 0.0f;
+    best_v = 0.0f;
+    float rt;
+    if (!slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2], best_t, &rt))
+        return;
+    int sp = 0;
+    stk.put(sp++, hdr.root_link, rt);
+    while (sp > 0) {
+        int2 e = stk.get(--sp);
+        float nt = __int_as_float(e.y);
+        if (nt > best_t) continue;
+        if (COUNT) cnt.pops++;
+        int link = e.x;
+        if (link < 0) {
+            int w = ~link;
+            int start = w >> 8, end = start + (w & 255);
+            for (int i = start; i < end; ++i) {
+                const float4* tp = reinterpret_cast<const float4*>(tris + i);
+                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                float u, v;
+                float t = tri_hit(ro, rd, mk(p0.x, p0.y, p0.z), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), &u, &v);
+                if (COUNT) cnt.tris++;
+                if (0.001f < t && t < best_t) {
+                    best_t = t;
+                    best_i = i;
+                    best_u = u;
+                    best_v = v;
+                }
+            }
+        } else {
+            const float4* p = reinterpret_cast<const float4*>(nodes + link);
+            float4 a = p[0], b = p[1], c = p[2];
+            int4 d = reinterpret_cast<const int4*>(p)[3];
+            float lt, rt2;
+            bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, best_t, &lt);
+            bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, best_t, &rt2);
+            if (lh && rh) {
+                if (lt < rt2) {
+                    stk.put(sp++, d.y, rt2);
+                    stk.put(sp++, d.x, lt);
+                } else {
+                    stk.put(sp++, d.x, lt);
+                    stk.put(sp++, d.y, rt2);
+                }
+            } else if (lh) {
+                stk.put(sp++, d.x, lt);
+            } else if (rh) {
+                stk.put(sp++, d.y, rt2);
+            }
+        }
+    }
+}
+
+// Any hit before max_t: rendering/taichi/intersect.py:155-219 (returns on
+// the first occluder; the boolean does not depend on the visit order).
+template <int STRIDE, bool COUNT>
+__device__ __forceinline__ bool bvh_occluded(const rx_node* __restrict__ nodes, const rx_tri* __restrict__ tris,
+                                             const rx_bvh_header& hdr, f3 ro, f3 rd, float max_t,
+                                             lane_stack<STRIDE> stk, trav_count& cnt) {
+    f3 inv = inv_dir(rd);
+    float rt;
+    if (!slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2], max_t, &rt))
+        return false;
+    int sp = 0;
+    stk.put(sp++, hdr.root_link, rt);
+    while (sp > 0) {
+        int2 e = stk.get(--sp);
+        float nt = __int_as_float(e.y);
+        if (nt > max_t) continue;
+        if (COUNT) cnt.pops++;
+        int link = e.x;
+        if (link < 0) {
+            int w = ~link;
+            int start = w >> 8, end = start + (w & 255);
+            for (int i = start; i < end; ++i) {
+                const float4* tp = reinterpret_cast<const float4*>(tris + i);
+                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                float u, v;
+                float t = tri_hit(ro, rd, mk(p0.x, p0.y, p0.z), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), &u, &v);
+                if (COUNT) cnt.tris++;
+                if (0.001f < t && t < max_t) return true;
+            }
+        } else {
+            const float4* p = reinterpret_cast<const float4*>(nodes + link);
+            float4 a = p[0], b = p[1], c = p[2];
+            int4 d = reinterpret_cast<const int4*>(p)[3];
+            float lt, rt2;
+            bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, max_t, &lt);
+            bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, max_t, &rt2);
+            if (lh && rh) {
+                if (lt < rt2) {
+                    stk.put(sp++, d.y, rt2);
+                    stk.put(sp++, d.x, lt);
+                } else {
+                    stk.put(sp++, d.x, lt);
+                    stk.put(sp++, d.y, rt2);
+                }
+            } else if (lh) {
+                stk.put(sp++, d.x, lt);
+            } else if (rh) {
+                stk.put(sp++, d.y, rt2);
+            }
+        }
+    }
+    return false;
+}
+
+// Result of scene_intersect (rendering/taichi/intersect.py:249-309) before
+// shading: the analytic primitives are scanned first with strict
+// "closer than" acceptance, then the BVH wins only if strictly closer.
+struct hit_rec {
+    float t;
+    int id;
+    int kind;
+    float u, v;
+};
+
+template <int STRIDE, bool COUNT>
+__device__ __forceinline__ hit_rec scene_closest(const rx_scene& sc, const rx_bvh_header& hdr, f3 ro, f3 rd,
+                                                 lane_stack<STRIDE> stk, trav_count& cnt) {
+    hit_rec h{1e9f, -1, RX_KIND_MISS, 0.0f, 0.0f};
+    for (int i = 0; i < sc.n_prims; ++i) {
+        float t = prim_hit(sc.prims[i], ro, rd);
+        if (COUNT) cnt.prims++;
+        if (t > 0.001f && t < h.t) {
+            h.t = t;
+            h.id = i;
+            h.kind = RX_KIND_PRIM;
+        }
+    }
+    if (sc.n_tris > 0) {
+        float t, u, v;
+        int idx;
+        bvh_closest<STRIDE, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, stk, t, idx, u, v, cnt);
+        if (0.001f < t && t < h.t) {
+            h.t = t;
+            h.id = idx;
+            h.kind = RX_KIND_BVH;
+            h.u = u;
+            h.v = v;
+        }
+    }
+    return h;
+}
+
+// rendering/taichi/intersect.py:222-246
+template <int STRIDE, bool COUNT>
+__device__ __forceinline__ bool scene_occluded(const rx_scene& sc, const rx_bvh_header& hdr, f3 ro, f3 rd,
+                                               float max_t, lane_stack<STRIDE> stk, trav_count& cnt) {
+    for (int i = 0; i < sc.n_prims; ++i) {
+        float t = prim_hit(sc.prims[i], ro, rd);
+        if (COUNT) cnt.prims++;
+        if (0.001f < t && t < max_t) return true;
+    }
+    if (sc.n_tris > 0) return bvh_occluded<STRIDE, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, max_t, stk, cnt);
+    return false;
+}
+
+// ---------------------------------------------------------------- wave helpers
+// Sum of a per-lane value over the 64 lanes of a wave.  Call only where
+// every lane of the wave is active (kernels keep out-of-range lanes alive
+// and idle instead of returning early).
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Adds a per-lane count to *dst with one atomic per wave.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long x) {
+    unsigned long long s = wave_sum_u64(x);
+    if ((threadIdx.x & 63) == 0 && s != 0ull) atomicAdd(dst, s);
+}
+
+}  // namespace rx

@@ -1,0 +1,699 @@
+// rx_render.hip — wavefront path tracer reproducing render_kernel's
+// estimator (rendering/taichi/trace.py:139-283).
+//
+// A pass traces `ns` samples of every pixel at once, as P = ns * W * H
+// paths (path p = sample (p / (W*H)), pixel (p % (W*H))).  Per bounce:
+//   extend   closest hit for every queued ray (SoA queue in HBM)
+//   shade    miss/emission/material, next-event request, BSDF sample,
+//            Russian roulette; survivors and next-event requests are
+//            compacted into the next queues with one atomic per wave
+//            (ballot + popcount)
+//   connect  sphere-light sampling and any-hit shadow rays for each
+//            next-event request, added to the path's colour
+// then accumulate folds the pass's samples into the running means in
+// sample order.  Randomness is Philox4x32-10 keyed by the seed and
+// counted by (pixel, sample, bounce, draw), so a pixel's samples do not
+// depend on the pass size, the queue order or what else is in flight.
+#include "rx_launch.h"
+
+namespace {
+
+using namespace rx;
+
+constexpr int BS = 256;
+
+// Path state and queues, SoA over the workspace.
+struct ws_view {
+    // per path
+    float *thr_x, *thr_y, *thr_z, *col_x, *col_y, *col_z, *ior;
+    float *fn_x, *fn_y, *fn_z, *fa_x, *fa_y, *fa_z, *fd;
+    // two ray queues
+    int32_t* q_path[2];
+    float *q_ox[2], *q_oy[2], *q_oz[2], *q_dx[2], *q_dy[2], *q_dz[2];
+    // hits of the current queue
+    float* h_t;
+    int32_t *h_id, *h_kind;
+    float *h_u, *h_v;
+    // next-event requests
+    int32_t* e_path;
+    float *e_px, *e_py, *e_pz, *e_nx, *e_ny, *e_nz, *e_ax, *e_ay, *e_az, *e_tx, *e_ty, *e_tz, *e_scale;
+    // queue lengths: [0], [1] ray queues, [2] next-event queue
+    int32_t* counts;
+};
+
+void carve(rx_arena& ar, long long P, ws_view& w) {
+    float** pf[] = {&w.thr_x, &w.thr_y, &w.thr_z, &w.col_x, &w.col_y, &w.col_z, &w.ior,
+                    &w.fn_x,  &w.fn_y,  &w.fn_z,  &w.fa_x,  &w.fa_y,  &w.fa_z,  &w.fd};
+    for (float** f : pf) *f = ar.take<float>(P);
+    for (int b = 0; b < 2; ++b) {
+        w.q_path[b] = ar.take<int32_t>(P);
+        w.q_ox[b] = ar.take<float>(P);
+        w.q_oy[b] = ar.take<float>(P);
+        w.q_oz[b] = ar.take<float>(P);
+        w.q_dx[b] = ar.take<float>(P);
+        w.q_dy[b] = ar.take<float>(P);
+        w.q_dz[b] = ar.take<float>(P);
+    }
+    w.h_t = ar.take<float>(P);
+    w.h_id = ar.take<int32_t>(P);
+    w.h_kind = ar.take<int32_t>(P);
+    w.h_u = ar.take<float>(P);
+    w.h_v = ar.take<float>(P);
+    w.e_path = ar.take<int32_t>(P);
+    float** pe[] = {&w.e_px, &w.e_py, &w.e_pz, &w.e_nx, &w.e_ny, &w.e_nz, &w.e_ax,
+                    &w.e_ay, &w.e_az, &w.e_tx, &w.e_ty, &w.e_tz, &w.e_scale};
+    for (float** f : pe) *f = ar.take<float>(P);
+    w.counts = ar.take<int32_t>(8);
+}
+
+struct pass_info {
+    int npix;       // W * H
+    int first;      // first sample index of the pass
+    int ns;         // samples in the pass
+    int depth;      // bounce index of the current launch
+};
+
+__device__ __forceinline__ u4 rng(const rx_render_params& prm, int pix, int frame, int depth, uint32_t draw) {
+    return philox4x32_10(u4{(uint32_t)pix, (uint32_t)frame, (uint32_t)depth, draw}, prm.seed_lo, prm.seed_hi);
+}
+
+// Appends to a queue: every lane of the wave calls this with `want`; the
+// wave reserves popcount(ballot) slots with one atomic and each wanting
+// lane gets base + its rank among the wanting lanes.
+__device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
+    unsigned long long m = __ballot(want);
+    int lane = threadIdx.x & 63;
+    int total = __popcll(m);
+    int base = 0;
+    if (lane == 0 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    unsigned long long below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    return base + __popcll(below);
+}
+
+// ---------------------------------------------------------------- generate
+__global__ __launch_bounds__(BS) void k_generate(rx_camera cam, pass_info pi, ws_view w) {
+    long long P = (long long)pi.npix * pi.ns;
+    int stride = gridDim.x * BS;
+    for (long long base = (long long)xcd_block() * BS; base < P; base += stride) {
+        long long p = base + threadIdx.x;
+        if (p < P) {
+            int s = (int)(p / pi.npix), pix = (int)(p % pi.npix);
+            int x = pix % cam.width, y = pix / cam.width;
+            f3 rd = camera_dir(cam, x, y, pi.first + s);
+            w.q_path[0][p] = (int32_t)p;
+            w.q_ox[0][p] = cam.pos[0];
+            w.q_oy[0][p] = cam.pos[1];
+            w.q_oz[0][p] = cam.pos[2];
+            w.q_dx[0][p] = rd.x;
+            w.q_dy[0][p] = rd.y;
+            w.q_dz[0][p] = rd.z;
+            w.thr_x[p] = 1.0f;
+            w.thr_y[p] = 1.0f;
+            w.thr_z[p] = 1.0f;
+            w.col_x[p] = 0.0f;
+            w.col_y[p] = 0.0f;
+            w.col_z[p] = 0.0f;
+            w.ior[p] = 1.0f;
+            w.fn_x[p] = 0.0f;
+            w.fn_y[p] = 0.0f;
+            w.fn_z[p] = 0.0f;
+            w.fa_x[p] = 0.0f;
+            w.fa_y[p] = 0.0f;
+            w.fa_z[p] = 0.0f;
+            w.fd[p] = 0.0f;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.counts[0] = (int32_t)P;
+        w.counts[1] = 0;
+        w.counts[2] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- extend
+template <int STACK, int SBS, bool COUNT>
+__global__ __launch_bounds__(SBS) void k_extend(rx_scene sc, ws_view w, int qi, rx_counters* __restrict__ ctr) {
+    __shared__ int2 stack_mem[STACK * SBS];
+    lane_stack<SBS> stk{stack_mem + threadIdx.x};
+    rx_bvh_header hdr;
+    if (sc.n_tris > 0) hdr = *sc.header;
+    const int n = w.counts[qi];
+    // the queue this bounce's shade fills and the next-event queue start empty
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.counts[1 - qi] = 0;
+        w.counts[2] = 0;
+    }
+    trav_count cnt{0, 0, 0};
+    unsigned long long rays = 0;
+    const int stride = gridDim.x * SBS;
+    for (int base = xcd_block() * SBS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
+            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+            hit_rec h = scene_closest<SBS, COUNT>(sc, hdr, ro, rd, stk, cnt);
+            w.h_t[i] = h.t;
+            w.h_id[i] = h.id;
+            w.h_kind[i] = h.kind;
+            w.h_u[i] = h.u;
+            w.h_v[i] = h.v;
+            rays++;
+        }
+    }
+    wave_add(&ctr->rays, rays);
+    if (COUNT) {
+        wave_add(&ctr->node_pops, cnt.pops);
+        wave_add(&ctr->tri_tests, cnt.tris);
+        wave_add(&ctr->prim_tests, cnt.prims);
+    }
+}
+
+// ---------------------------------------------------------------- shade helpers
+// rendering/taichi/scatter.py:14-30 (cosine_hemisphere) and :33-38
+__device__ __forceinline__ f3 diffuse_dir(f3 n, float r1, float r2) {
+    float sp, cp;
+    sincos_2pi(r1, &sp, &cp);
+    float r = sqrtf(r2);
+    float lx = cp * r, ly = sp * r, lz = sqrtf(fmaxf(0.0f, 1.0f - r2));
+    f3 helper = fabsf(n.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    f3 b = normalize(cross(n, helper));
+    f3 t = cross(b, n);
+    f3 s = normalize((t * lx + b * ly) + n * lz);
+    if (dot(s, s) < 1e-8f) s = n;
+    return normalize(s);
+}
+
+__device__ __forceinline__ f3 reflect(f3 d, f3 n) { return d - n * (2.0f * dot(d, n)); }
+
+// rendering/taichi/scatter.py:6-11 with cos(theta) = 1 - 2 xi directly
+// (theta = acos(1 - 2 xi) there) and sin(theta) = sqrt(1 - cos^2).
+__device__ __forceinline__ f3 unit_vector(float r1, float r2) {
+    float z = 1.0f - 2.0f * r1;
+    float st = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    sincos_2pi(r2, &sp, &cp);
+    return mk(st * cp, st * sp, z);
+}
+
+// rendering/taichi/scatter.py:50-54
+__device__ __forceinline__ f3 metal_dir(f3 d, f3 n, float rough, float r1, float r2) {
+    return normalize(reflect(d, n) + unit_vector(r1, r2) * rough);
+}
+
+__device__ __forceinline__ float pow5(float x) {
+    float x2 = x * x;
+    return (x2 * x2) * x;
+}
+
+// rendering/taichi/scatter.py:40-49, :57-60, :71-81
+__device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
+    float cos_t = fminf(-dot(d, n), 1.0f);
+    float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    bool cannot = eta * sin_t > 1.0f;
+    float r0 = (1.0f - eta) / (1.0f + eta);
+    r0 = r0 * r0;
+    float sch = r0 + (1.0f - r0) * pow5(1.0f - cos_t);
+    if (cannot || sch > xi) return reflect(d, n);
+    f3 perp = (d + n * cos_t) * eta;
+    f3 par = n * (-sqrtf(fabsf(1.0f - dot(perp, perp))));
+    return perp + par;
+}
+
+// ---------------------------------------------------------------- shade
+__global__ __launch_bounds__(BS) void k_shade(rx_scene sc, rx_render_params prm, pass_info pi, ws_view w, int qi) {
+    const int n = w.counts[qi];
+    const int qo = 1 - qi;
+    const int depth = pi.depth;
+    const bool last = depth + 1 >= prm.max_depth;
+    const bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
+    const int stride = gridDim.x * BS;
+    for (int base = xcd_block() * BS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        bool valid = i < n;
+        bool cont = false, want_nee = false;
+        int p = 0;
+        f3 hit_p = mk(0.0f, 0.0f, 0.0f), nrm = hit_p, alb = hit_p, thr0 = hit_p, ro2 = hit_p, rd2 = hit_p;
+        float scale = 1.0f;
+        if (valid) {
+            p = w.q_path[qi][i];
+            int pix = p % pi.npix, frame = pi.first + p / pi.npix;
+            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
+            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+            float t = w.h_t[i];
+            int kind = w.h_kind[i], id = w.h_id[i];
+            f3 thr = mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
+            f3 col = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+            if (t < 0.0f || t >= 1e9f) {
+                // trace.py:154-159
+                f3 sky;
+                if (prm.use_sky) {
+                    float s = 0.5f * (rd.y + 1.0f);
+                    sky = mk(1.0f, 1.0f, 1.0f) * (1.0f - s) + mk(0.5f, 0.7f, 1.0f) * s;
+                } else {
+                    sky = ld3(prm.bg);
+                }
+                col = col + thr * sky;
+            } else {
+                rx_material m;
+                f3 nn;
+                if (kind == RX_KIND_BVH) {
+                    const rx_tri_shade& sh = sc.tri_shade[id];
+                    float u = w.h_u[i], v = w.h_v[i];
+                    float ww = (1.0f - u) - v;
+                    nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
+                    m = sc.materials[sh.mat];
+                } else {
+                    const rx_prim pr = sc.prims[id];
+                    nn = prim_normal(pr, ro, rd, t);
+                    m = sc.prim_materials[id];
+                }
+                // trace.py:161-165
+                bool front = dot(nn, rd) < 0.0f;
+                if (!front) nn = -nn;
+                hit_p = ro + rd * t;
+                f3 albedo = ld3(m.albedo);
+                if (depth == 0) {
+                    w.fn_x[p] = nn.x;
+                    w.fn_y[p] = nn.y;
+                    w.fn_z[p] = nn.z;
+                    w.fa_x[p] = albedo.x;
+                    w.fa_y[p] = albedo.y;
+                    w.fa_z[p] = albedo.z;
+                    w.fd[p] = t;
+                }
+                if (m.type == RX_MAT_EMISSIVE) {
+                    col = col + (thr * albedo) * m.emission;
+                } else {
+                    u4 r0 = rng(prm, pix, frame, depth, 0u);
+                    u4 r1 = rng(prm, pix, frame, depth, 1u);
+                    f3 nd = rd;
+                    if (m.type == RX_MAT_DIFFUSE) {
+                        if (nee_on) {
+                            want_nee = true;
+                            scale = 1.0f;
+                        }
+                        thr0 = thr;
+                        nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
+                        thr = thr * albedo;
+                    } else if (m.type == RX_MAT_METAL) {
+                        nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
+                        thr = thr * albedo;
+                    } else if (m.type == RX_MAT_GLOSSY) {
+                        if (nee_on) {
+                            want_nee = true;
+                            scale = m.roughness;
+                        }
+                        thr0 = thr;
+                        if (u01(r0.w) < m.roughness)
+                            nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
+                        else
+                            nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
+                        thr = thr * albedo;
+                    } else if (m.type == RX_MAT_DIELECTRIC) {
+                        float cur = w.ior[p];
+                        float eta;
+                        if (front) {
+                            eta = cur / m.ior;
+                            cur = m.ior;
+                        } else {
+                            eta = cur / 1.0f;
+                            cur = 1.0f;
+                        }
+                        w.ior[p] = cur;
+                        nd = dielectric_dir(rd, nn, eta, u01(r0.w));
+                        thr = thr * albedo;
+                    }
+                    nrm = nn;
+                    alb = albedo;
+                    // trace.py:227-234
+                    bool alive = true;
+                    if (depth > 2) {
+                        float rr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                        if (u01(r1.z) > rr)
+                            alive = false;
+                        else
+                            thr = thr / rr;
+                    }
+                    if (alive && !last) {
+                        cont = true;
+                        ro2 = hit_p + nn * 1e-4f;
+                        rd2 = nd;
+                    }
+                    w.thr_x[p] = thr.x;
+                    w.thr_y[p] = thr.y;
+                    w.thr_z[p] = thr.z;
+                }
+            }
+            w.col_x[p] = col.x;
+            w.col_y[p] = col.y;
+            w.col_z[p] = col.z;
+        }
+        int slot = wave_append(&w.counts[qo], cont);
+        if (cont) {
+            w.q_path[qo][slot] = p;
+            w.q_ox[qo][slot] = ro2.x;
+            w.q_oy[qo][slot] = ro2.y;
+            w.q_oz[qo][slot] = ro2.z;
+            w.q_dx[qo][slot] = rd2.x;
+            w.q_dy[qo][slot] = rd2.y;
+            w.q_dz[qo][slot] = rd2.z;
+        }
+        int es = wave_append(&w.counts[2], want_nee);
+        if (want_nee) {
+            w.e_path[es] = p;
+            w.e_px[es] = hit_p.x;
+            w.e_py[es] = hit_p.y;
+            w.e_pz[es] = hit_p.z;
+            w.e_nx[es] = nrm.x;
+            w.e_ny[es] = nrm.y;
+            w.e_nz[es] = nrm.z;
+            w.e_ax[es] = alb.x;
+            w.e_ay[es] = alb.y;
+            w.e_az[es] = alb.z;
+            w.e_tx[es] = thr0.x;
+            w.e_ty[es] = thr0.y;
+            w.e_tz[es] = thr0.z;
+            w.e_scale[es] = scale;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- connect
+// rendering/taichi/trace.py:23-57 (_sphere_light_sample)
+__device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, float xi1, float xi2, f3* dir,
+                                                    float* t_light, float* solid) {
+    f3 tc = c - hp;
+    float dist2 = fmaxf(dot(tc, tc), 1e-6f);
+    float dist = sqrtf(dist2);
+    f3 axis = tc / dist;
+    float sin2 = fminf(radius * radius / dist2, 1.0f);
+    float cos_max = sqrtf(fmaxf(0.0f, 1.0f - sin2));
+    float cos_t = 1.0f - xi1 * (1.0f - cos_max);
+    float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    float sp, cp;
+    sincos_2pi(xi2, &sp, &cp);
+    f3 helper = fabsf(axis.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    f3 b = normalize(cross(axis, helper));
+    f3 t = cross(b, axis);
+    f3 ld = normalize((t * (cp * sin_t) + b * (sp * sin_t)) + axis * cos_t);
+    f3 oc = hp - c;
+    float half_b = dot(oc, ld);
+    float cc = dot(oc, oc) - radius * radius;
+    float disc = fmaxf(half_b * half_b - cc, 0.0f);
+    *t_light = -half_b - sqrtf(disc);
+    *solid = 2.0f * (1.0f - cos_max);
+    *dir = ld;
+}
+
+// rendering/taichi/trace.py:60-93 (direct_light_sample) for every queued
+// next-event request, then colour += (throughput * direct) * scale
+// (scale = 1 for diffuse, roughness for glossy, trace.py:199-212).
+template <int STACK, int SBS, bool COUNT>
+__global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params prm, pass_info pi, ws_view w,
+                                                 rx_counters* __restrict__ ctr) {
+    __shared__ int2 stack_mem[STACK * SBS];
+    lane_stack<SBS> stk{stack_mem + threadIdx.x};
+    rx_bvh_header hdr;
+    if (sc.n_tris > 0) hdr = *sc.header;
+    const int n = w.counts[2];
+    const int L = sc.n_lights;
+    const bool all = prm.direct_light_mode == 1;
+    trav_count cnt{0, 0, 0};
+    unsigned long long shadow = 0;
+    const int stride = gridDim.x * SBS;
+    for (int base = xcd_block() * SBS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            int p = w.e_path[i];
+            int pix = p % pi.npix, frame = pi.first + p / pi.npix;
+            f3 hp = mk(w.e_px[i], w.e_py[i], w.e_pz[i]);
+            f3 nn = mk(w.e_nx[i], w.e_ny[i], w.e_nz[i]);
+            f3 alb = mk(w.e_ax[i], w.e_ay[i], w.e_az[i]);
+            u4 r0 = rng(prm, pix, frame, pi.depth, 0u);
+            int sel = (int)(u01(r0.x) * (float)L);
+            if (sel > L - 1) sel = L - 1;
+            f3 direct = mk(0.0f, 0.0f, 0.0f);
+            for (int li = 0; li < L; ++li) {
+                if (!all && li != sel) continue;
+                float xi1, xi2;
+                if (all) {
+                    u4 rl = rng(prm, pix, frame, pi.depth, 2u + (uint32_t)li);
+                    xi1 = u01(rl.x);
+                    xi2 = u01(rl.y);
+                } else {
+                    xi1 = u01(r0.y);
+                    xi2 = u01(r0.z);
+                }
+                const rx_light lt = sc.lights[li];
+                f3 ld;
+                float t_light, solid;
+                sphere_light_sample(hp, ld3(lt.center), lt.radius, xi1, xi2, &ld, &t_light, &solid);
+                float ndotl = dot(nn, ld);
+                if (ndotl > 0.0f) {
+                    float max_t = t_light * 0.999f;
+                    if (max_t > 0.001f) {
+                        f3 so = hp + nn * 1e-4f;
+                        shadow++;
+                        if (!scene_occluded<SBS, COUNT>(sc, hdr, so, ld, max_t, stk, cnt)) {
+                            float wgt = all ? 1.0f : (float)L;
+                            direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
+                        }
+                    }
+                }
+            }
+            f3 thr = mk(w.e_tx[i], w.e_ty[i], w.e_tz[i]);
+            f3 add = (thr * direct) * w.e_scale[i];
+            w.col_x[p] = w.col_x[p] + add.x;
+            w.col_y[p] = w.col_y[p] + add.y;
+            w.col_z[p] = w.col_z[p] + add.z;
+        }
+    }
+    wave_add(&ctr->rays, shadow);
+    wave_add(&ctr->shadow_rays, shadow);
+    if (COUNT) {
+        wave_add(&ctr->shadow_node_pops, cnt.pops);
+        wave_add(&ctr->shadow_tri_tests, cnt.tris);
+        wave_add(&ctr->shadow_prim_tests, cnt.prims);
+    }
+}
+
+// ---------------------------------------------------------------- accumulate
+// rendering/taichi/trace.py:262-283: clamp, running mean of colour and the
+// first-hit guides in sample order, display = sqrt(min(acc, 1)).
+__global__ __launch_bounds__(BS) void k_accumulate(rx_render_params prm, pass_info pi, ws_view w, rx_accum acc) {
+    const int stride = gridDim.x * BS;
+    for (int base = xcd_block() * BS; base < pi.npix; base += stride) {
+        int pix = base + threadIdx.x;
+        if (pix >= pi.npix) continue;
+        f3 c = ld3(acc.color + 3 * pix), nn = ld3(acc.normal + 3 * pix), aa = ld3(acc.albedo + 3 * pix);
+        float dd = acc.depth[pix];
+        for (int s = 0; s < pi.ns; ++s) {
+            long long p = (long long)s * pi.npix + pix;
+            int frame = pi.first + s;
+            f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+            if (prm.sample_clamp > 0.0f) {
+                float lim = prm.sample_clamp;
+                smp = mk(fminf(fmaxf(smp.x, 0.0f), lim), fminf(fmaxf(smp.y, 0.0f), lim),
+                         fminf(fmaxf(smp.z, 0.0f), lim));
+            }
+            f3 ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
+            f3 as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
+            float ds = w.fd[p];
+            if (frame == 0) {
+                c = smp;
+                nn = ns_;
+                aa = as_;
+                dd = ds;
+            } else {
+                float f = (float)frame, f1 = (float)(frame + 1);
+                c = (c * f + smp) / f1;
+                nn = (nn * f + ns_) / f1;
+                aa = (aa * f + as_) / f1;
+                dd = (dd * f + ds) / f1;
+            }
+        }
+        acc.color[3 * pix] = c.x;
+        acc.color[3 * pix + 1] = c.y;
+        acc.color[3 * pix + 2] = c.z;
+        acc.normal[3 * pix] = nn.x;
+        acc.normal[3 * pix + 1] = nn.y;
+        acc.normal[3 * pix + 2] = nn.z;
+        acc.albedo[3 * pix] = aa.x;
+        acc.albedo[3 * pix + 1] = aa.y;
+        acc.albedo[3 * pix + 2] = aa.z;
+        acc.depth[pix] = dd;
+        if (acc.display) {
+            acc.display[3 * pix] = sqrtf(fminf(c.x, 1.0f));
+            acc.display[3 * pix + 1] = sqrtf(fminf(c.y, 1.0f));
+            acc.display[3 * pix + 2] = sqrtf(fminf(c.z, 1.0f));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- stage timing
+struct stage_timer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    static constexpr int MAXE = 512;
+    hipEvent_t ev[MAXE];
+    int stage[MAXE / 2];
+    int n = 0;
+    int begin(int st) {
+        if (!on) return RX_OK;
+        if (n + 2 > MAXE) return rx::set_error(RX_ERR_INVALID, "too many profiled launches in one call");
+        RX_TRY_HIP(hipEventCreate(&ev[n]));
+        RX_TRY_HIP(hipEventCreate(&ev[n + 1]));
+        stage[n / 2] = st;
+        RX_TRY_HIP(hipEventRecord(ev[n], s));
+        return RX_OK;
+    }
+    int end() {
+        if (!on) return RX_OK;
+        RX_TRY_HIP(hipEventRecord(ev[n + 1], s));
+        n += 2;
+        return RX_OK;
+    }
+    int finish(rx_profile* prof) {
+        if (!on) return RX_OK;
+        RX_TRY_HIP(hipStreamSynchronize(s));
+        float sum[5] = {0, 0, 0, 0, 0};
+        int cnt[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < n; k += 2) {
+            float ms = 0.0f;
+            RX_TRY_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            sum[stage[k / 2]] += ms;
+            cnt[stage[k / 2]]++;
+            (void)hipEventDestroy(ev[k]);
+            (void)hipEventDestroy(ev[k + 1]);
+        }
+        n = 0;
+        if (prof) {
+            prof->ms_generate += sum[0];
+            prof->ms_extend += sum[1];
+            prof->ms_shade += sum[2];
+            prof->ms_connect += sum[3];
+            prof->ms_accumulate += sum[4];
+            prof->n_extend += cnt[1];
+            prof->n_connect += cnt[3];
+        }
+        return RX_OK;
+    }
+    ~stage_timer() {
+        for (int k = 0; k < n; ++k) (void)hipEventDestroy(ev[k]);
+    }
+};
+
+#define RX_STAGE(timer, st, launch)       \
+    do {                                  \
+        int rx_s_ = (timer).begin(st);    \
+        if (rx_s_) return rx_s_;          \
+        launch;                           \
+        RX_CHECK_LAUNCH();                \
+        rx_s_ = (timer).end();            \
+        if (rx_s_) return rx_s_;          \
+    } while (0)
+
+template <int STACK, int SBS>
+int run_pass(const rx_scene& sc, const rx_camera& cam, const rx_render_params& prm, int first, int ns, ws_view& w,
+             const rx_accum& acc, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+    pass_info pi{cam.width * cam.height, first, ns, 0};
+    long long P = (long long)pi.npix * ns;
+    bool count = (prm.flags & RX_FLAG_COUNT_TRAFFIC) != 0;
+    RX_STAGE(tm, 0, (k_generate<<<grid_for(P, BS, 16), BS, 0, s>>>(cam, pi, w)));
+    int gt = grid_for(P, SBS, SBS == 256 ? 16 : 32);
+    int gs = grid_for(P, BS, 16);
+    for (int d = 0; d < prm.max_depth; ++d) {
+        int qi = d & 1;
+        pi.depth = d;
+        if (count)
+            RX_STAGE(tm, 1, (k_extend<STACK, SBS, true><<<gt, SBS, 0, s>>>(sc, w, qi, ctr)));
+        else
+            RX_STAGE(tm, 1, (k_extend<STACK, SBS, false><<<gt, SBS, 0, s>>>(sc, w, qi, ctr)));
+        RX_STAGE(tm, 2, (k_shade<<<gs, BS, 0, s>>>(sc, prm, pi, w, qi)));
+        if (d < prm.direct_light_max_depth && sc.n_lights > 0) {
+            if (count)
+                RX_STAGE(tm, 3, (k_connect<STACK, SBS, true><<<gt, SBS, 0, s>>>(sc, prm, pi, w, ctr)));
+            else
+                RX_STAGE(tm, 3, (k_connect<STACK, SBS, false><<<gt, SBS, 0, s>>>(sc, prm, pi, w, ctr)));
+        }
+    }
+    RX_STAGE(tm, 4, (k_accumulate<<<grid_for(pi.npix, BS, 16), BS, 0, s>>>(prm, pi, w, acc)));
+    return RX_OK;
+}
+
+}  // namespace
+
+extern "C" int rx_render_workspace_size(int width, int height, int samples_per_pass, int n_lights, size_t* bytes) {
+    (void)n_lights;
+    RX_REQUIRE(bytes, RX_ERR_INVALID, "bytes is NULL");
+    RX_REQUIRE(width > 0 && height > 0 && samples_per_pass > 0, RX_ERR_INVALID,
+               "width, height and samples_per_pass must be positive");
+    long long P = (long long)width * height * samples_per_pass;
+    RX_REQUIRE(P <= (1LL << 31) - 1, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
+    rx_arena ar{nullptr, 0, 0};
+    ws_view w;
+    carve(ar, P, w);
+    *bytes = ar.off + 256;
+    return RX_OK;
+}
+
+extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const rx_render_params* params,
+                         int first_sample, int n_samples, void* d_workspace, size_t workspace_bytes,
+                         const rx_accum* accum, rx_counters* d_counters, rx_profile* profile, void* stream) {
+    RX_REQUIRE(scene && camera && params && accum && d_counters, RX_ERR_INVALID, "NULL argument");
+    RX_REQUIRE(camera->width > 0 && camera->height > 0, RX_ERR_INVALID, "camera size must be positive");
+    RX_REQUIRE(first_sample >= 0 && n_samples >= 0, RX_ERR_INVALID, "sample range must be non-negative");
+    RX_REQUIRE(params->samples_per_pass > 0, RX_ERR_INVALID, "samples_per_pass must be positive");
+    RX_REQUIRE(params->max_depth >= 0, RX_ERR_INVALID, "max_depth must be >= 0");
+    RX_REQUIRE(params->direct_light_mode == 0 || params->direct_light_mode == 1, RX_ERR_INVALID,
+               "direct_light_mode must be 'one' or 'all'");
+    RX_REQUIRE(accum->color && accum->normal && accum->albedo && accum->depth, RX_ERR_INVALID,
+               "accumulation buffers must be given");
+    RX_REQUIRE(scene->n_prims >= 0 && scene->n_tris >= 0 && scene->n_lights >= 0, RX_ERR_INVALID,
+               "negative scene size");
+    RX_REQUIRE(scene->n_prims == 0 || (scene->prims && scene->prim_materials), RX_ERR_INVALID,
+               "scene has primitives but no primitive arrays");
+    RX_REQUIRE(scene->n_lights == 0 || scene->lights, RX_ERR_INVALID, "scene has lights but lights is NULL");
+    if (scene->n_tris > 0) {
+        RX_REQUIRE(scene->nodes && scene->header && scene->tris && scene->tri_shade && scene->materials,
+                   RX_ERR_INVALID, "scene has triangles but no BVH/shading arrays");
+        RX_REQUIRE(scene->bvh_depth >= 0 && scene->bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY,
+                   "BVH depth %d exceeds %d", scene->bvh_depth, RX_MAX_BVH_DEPTH);
+    }
+    if (n_samples == 0) return RX_OK;
+    int spp = params->samples_per_pass;
+    size_t need = 0;
+    int st = rx_render_workspace_size(camera->width, camera->height, spp, scene->n_lights, &need);
+    if (st) return st;
+    RX_REQUIRE(d_workspace && workspace_bytes >= need, RX_ERR_WORKSPACE,
+               "render workspace too small: need %zu bytes, have %zu", need, workspace_bytes);
+    rx_arena ar{reinterpret_cast<char*>(d_workspace), workspace_bytes, 0};
+    ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
+    ws_view w;
+    carve(ar, (long long)camera->width * camera->height * spp, w);
+
+    hipStream_t s = rx_stream(stream);
+    stage_timer tm;
+    tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
+    tm.s = s;
+    bool deep = scene->n_tris > 0 && use_deep_stack(scene->bvh_depth);
+    int done = 0;
+    while (done < n_samples) {
+        int ns = n_samples - done < spp ? n_samples - done : spp;
+        if (deep)
+            st = run_pass<STACK_DEEP, BS_DEEP>(*scene, *camera, *params, first_sample + done, ns, w, *accum,
+                                               d_counters, tm, s);
+        else
+            st = run_pass<STACK_SHALLOW, BS_SHALLOW>(*scene, *camera, *params, first_sample + done, ns, w, *accum,
+                                                     d_counters, tm, s);
+        if (st) return st;
+        if (profile) profile->n_passes += 1;
+        done += ns;
+        if (tm.on) {
+            st = tm.finish(profile);
+            if (st) return st;
+        }
+    }
+    return RX_OK;
+}

@@ -1,0 +1,137 @@
+// rx_trace.hip — batched closest-hit and any-hit queries over ray arrays:
+// rendering/taichi/intersect.py:249-309 (scene_intersect) and :222-246
+// (scene_occluded), one ray per lane, traversal stack in LDS.
+#include "rx_launch.h"
+
+namespace {
+
+using namespace rx;
+
+template <int STACK, int BS, bool COUNT>
+__global__ __launch_bounds__(BS) void k_closest(rx_scene sc, const float* __restrict__ rays, int n,
+                                                float* __restrict__ t_out, int32_t* __restrict__ id_out,
+                                                int32_t* __restrict__ kind_out, float* __restrict__ u_out,
+                                                float* __restrict__ v_out, rx_counters* __restrict__ ctr) {
+    __shared__ int2 stack_mem[STACK * BS];
+    lane_stack<BS> stk{stack_mem + threadIdx.x};
+    rx_bvh_header hdr;
+    if (sc.n_tris > 0) hdr = *sc.header;
+    trav_count cnt{0, 0, 0};
+    const int stride = gridDim.x * BS;
+    for (int base = xcd_block() * BS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            f3 ro = mk(rays[i], rays[n + i], rays[2 * n + i]);
+            f3 rd = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+            hit_rec h = scene_closest<BS, COUNT>(sc, hdr, ro, rd, stk, cnt);
+            if (t_out) t_out[i] = h.t;
+            if (id_out) id_out[i] = h.id;
+            if (kind_out) kind_out[i] = h.kind;
+            if (u_out) u_out[i] = h.u;
+            if (v_out) v_out[i] = h.v;
+        }
+    }
+    if (COUNT) {
+        wave_add(&ctr->node_pops, cnt.pops);
+        wave_add(&ctr->tri_tests, cnt.tris);
+        wave_add(&ctr->prim_tests, cnt.prims);
+    }
+}
+
+template <int STACK, int BS, bool COUNT>
+__global__ __launch_bounds__(BS) void k_any(rx_scene sc, const float* __restrict__ rays,
+                                            const float* __restrict__ tmax, int n, int32_t* __restrict__ occ,
+                                            rx_counters* __restrict__ ctr) {
+    __shared__ int2 stack_mem[STACK * BS];
+    lane_stack<BS> stk{stack_mem + threadIdx.x};
+    rx_bvh_header hdr;
+    if (sc.n_tris > 0) hdr = *sc.header;
+    trav_count cnt{0, 0, 0};
+    const int stride = gridDim.x * BS;
+    for (int base = xcd_block() * BS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            f3 ro = mk(rays[i], rays[n + i], rays[2 * n + i]);
+            f3 rd = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+            occ[i] = scene_occluded<BS, COUNT>(sc, hdr, ro, rd, tmax[i], stk, cnt) ? 1 : 0;
+        }
+    }
+    if (COUNT) {
+        wave_add(&ctr->shadow_node_pops, cnt.pops);
+        wave_add(&ctr->shadow_tri_tests, cnt.tris);
+        wave_add(&ctr->shadow_prim_tests, cnt.prims);
+    }
+}
+
+int check_scene(const rx_scene* sc) {
+    RX_REQUIRE(sc, RX_ERR_INVALID, "scene is NULL");
+    RX_REQUIRE(sc->n_prims >= 0 && sc->n_tris >= 0 && sc->n_lights >= 0, RX_ERR_INVALID, "negative scene size");
+    RX_REQUIRE(sc->n_prims == 0 || sc->prims, RX_ERR_INVALID, "scene has primitives but prims is NULL");
+    if (sc->n_tris > 0) {
+        RX_REQUIRE(sc->nodes && sc->header && sc->tris, RX_ERR_INVALID, "scene has triangles but no BVH");
+        RX_REQUIRE(sc->bvh_depth >= 0 && sc->bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY,
+                   "BVH depth %d exceeds %d", sc->bvh_depth, RX_MAX_BVH_DEPTH);
+    }
+    return RX_OK;
+}
+
+}  // namespace
+
+extern "C" int rx_closest_hit(const rx_scene* scene, const float* d_rays, int n, float* d_t, int32_t* d_id,
+                              int32_t* d_kind, float* d_u, float* d_v, rx_counters* d_counters, void* stream) {
+    int st = check_scene(scene);
+    if (st) return st;
+    RX_REQUIRE(n >= 0, RX_ERR_INVALID, "n must be >= 0");
+    if (n == 0) return RX_OK;
+    RX_REQUIRE(d_rays, RX_ERR_INVALID, "rays is NULL");
+    hipStream_t s = rx_stream(stream);
+    bool deep = scene->n_tris > 0 && use_deep_stack(scene->bvh_depth);
+    if (deep) {
+        int g = grid_for(n, BS_DEEP, 32);
+        if (d_counters)
+            k_closest<STACK_DEEP, BS_DEEP, true><<<g, BS_DEEP, 0, s>>>(*scene, d_rays, n, d_t, d_id, d_kind, d_u,
+                                                                      d_v, d_counters);
+        else
+            k_closest<STACK_DEEP, BS_DEEP, false><<<g, BS_DEEP, 0, s>>>(*scene, d_rays, n, d_t, d_id, d_kind, d_u,
+                                                                       d_v, d_counters);
+    } else {
+        int g = grid_for(n, BS_SHALLOW, 16);
+        if (d_counters)
+            k_closest<STACK_SHALLOW, BS_SHALLOW, true><<<g, BS_SHALLOW, 0, s>>>(*scene, d_rays, n, d_t, d_id, d_kind,
+                                                                               d_u, d_v, d_counters);
+        else
+            k_closest<STACK_SHALLOW, BS_SHALLOW, false><<<g, BS_SHALLOW, 0, s>>>(*scene, d_rays, n, d_t, d_id,
+                                                                                d_kind, d_u, d_v, d_counters);
+    }
+    RX_CHECK_LAUNCH();
+    return RX_OK;
+}
+
+extern "C" int rx_any_hit(const rx_scene* scene, const float* d_rays, const float* d_tmax, int n,
+                          int32_t* d_occluded, rx_counters* d_counters, void* stream) {
+    int st = check_scene(scene);
+    if (st) return st;
+    RX_REQUIRE(n >= 0, RX_ERR_INVALID, "n must be >= 0");
+    if (n == 0) return RX_OK;
+    RX_REQUIRE(d_rays && d_tmax && d_occluded, RX_ERR_INVALID, "NULL argument");
+    hipStream_t s = rx_stream(stream);
+    bool deep = scene->n_tris > 0 && use_deep_stack(scene->bvh_depth);
+    if (deep) {
+        int g = grid_for(n, BS_DEEP, 32);
+        if (d_counters)
+            k_any<STACK_DEEP, BS_DEEP, true><<<g, BS_DEEP, 0, s>>>(*scene, d_rays, d_tmax, n, d_occluded, d_counters);
+        else
+            k_any<STACK_DEEP, BS_DEEP, false><<<g, BS_DEEP, 0, s>>>(*scene, d_rays, d_tmax, n, d_occluded,
+                                                                   d_counters);
+    } else {
+        int g = grid_for(n, BS_SHALLOW, 16);
+        if (d_counters)
+            k_any<STACK_SHALLOW, BS_SHALLOW, true><<<g, BS_SHALLOW, 0, s>>>(*scene, d_rays, d_tmax, n, d_occluded,
+                                                                           d_counters);
+        else
+            k_any<STACK_SHALLOW, BS_SHALLOW, false><<<g, BS_SHALLOW, 0, s>>>(*scene, d_rays, d_tmax, n, d_occluded,
+                                                                            d_counters);
+    }
+    RX_CHECK_LAUNCH();
+    return RX_OK;
+}

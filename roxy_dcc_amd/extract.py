@@ -1,0 +1,126 @@
+"""Scene flattening: rendering/taichi/extractor.py:84-249.
+
+Walks the object tree exactly as ``_collect`` does (:134-205): a ``Mesh``
+(per-triangle objects) is transformed triangle by triangle, an
+``IndexedMesh`` through its vectorised ``indexed_triangle_arrays``, and any
+other geometry is exported once per object with ``taichi_export()``.
+Lights are the exported spheres with material type 3 and emission > 0
+(:36-40).  Per-triangle ``Mesh`` data is appended as one batch after the
+indexed batches (:223-224), as in the reference.
+
+``extract_scene`` returns the reference's statistics dict
+(:232-249); the device-resident scene it built rides along as the
+``scene`` attribute of that dict.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .bvh import TriangleBatch
+from .scene import DeviceScene, FlatScene, PrimitiveSlots
+
+
+class ExtractResult(dict):
+    """The statistics dict of extractor.py:232-249, plus ``.scene``."""
+
+    scene: DeviceScene = None
+    flat: FlatScene = None
+
+
+def _material_data(mat):
+    mt = mat.taichi_type_id()
+    params = mat.taichi_params()
+    if getattr(mat, "_albedo_texture", None) is not None:
+        raise NotImplementedError("image textures are not supported by this backend yet")
+    return {
+        "mat_type": mt,
+        "albedo": [float(c) for c in mat._albedo],
+        "roughness": params[0] if mt in (1, 4) else 0.0,
+        "ior": params[0] if mt == 2 else 1.0,
+        "emission": params[0] if mt == 3 else 0.0,
+        "texture_idx": -1,
+    }
+
+
+def _default_material():
+    from .world import Diffuse
+
+    return Diffuse((0.8, 0.8, 0.8))
+
+
+def _is_indexed(geo):
+    return hasattr(geo, "_tri_pos_idx") and hasattr(geo, "indexed_triangle_arrays")
+
+
+def _is_object_mesh(geo):
+    return hasattr(geo, "_triangles")
+
+
+def _uv2(uv):
+    return [float(uv[0]), float(uv[1])] if uv is not None else [0.0, 0.0]
+
+
+def flatten_scene(world) -> FlatScene:
+    """Host-side flattening of ``world`` (no device work)."""
+    prim_slots = []
+    tri_dicts = []
+    batches = []
+
+    def collect(obj):
+        if not obj.renderable:
+            return
+        for shape in obj.shapes:
+            geo = shape.geometry
+            if _is_object_mesh(geo):
+                m = obj.world_matrix
+                inv_t = obj.world_inverse_transpose_matrix
+                for tri in geo._triangles:
+                    def xn(n):
+                        v = np.asarray(list(inv_t.transform_vector(n)), dtype=np.float64)
+                        ln = float(np.sqrt(v @ v))
+                        return list(v / ln) if ln > 0 else [0.0, 0.0, 0.0]
+
+                    face = tri._normal
+                    mat = shape.material_for_group(tri._group) or _default_material()
+                    tri_dicts.append({
+                        "v0": list(m.transform_point(tri._v0)), "v1": list(m.transform_point(tri._v1)),
+                        "v2": list(m.transform_point(tri._v2)),
+                        "n0": xn(tri._n0 if tri._n0 is not None else face),
+                        "n1": xn(tri._n1 if tri._n1 is not None else face),
+                        "n2": xn(tri._n2 if tri._n2 is not None else face),
+                        "uv0": _uv2(tri._uv0), "uv1": _uv2(tri._uv1), "uv2": _uv2(tri._uv2),
+                        "has_uv": int(tri._uv0 is not None and tri._uv1 is not None and tri._uv2 is not None),
+                        **_material_data(mat),
+                    })
+            elif _is_indexed(geo):
+                arrays = geo.indexed_triangle_arrays(obj.world_matrix, obj.world_inverse_transpose_matrix)
+                by_group = {}
+                for gi, group in enumerate(arrays["groups"]):
+                    mat = shape.material_for_group(group) or _default_material()
+                    by_group[gi] = _material_data(mat)
+                batches.append(TriangleBatch.from_indexed_arrays(arrays, by_group))
+            else:
+                exported = obj.taichi_export()
+                if isinstance(exported, list):
+                    prim_slots.extend(exported)
+                elif exported:
+                    prim_slots.append(exported)
+                break  # taichi_export covers every shape of the object
+        for child in obj.children:
+            collect(child)
+
+    for obj in world.objects:
+        collect(obj)
+    if tri_dicts:
+        batches.append(TriangleBatch.from_dicts(tri_dicts))
+    return FlatScene(batches=batches, prims=PrimitiveSlots.from_dicts(prim_slots))
+
+
+def extract_scene(world, device="cuda", max_leaf=None, method=None) -> ExtractResult:
+    """Flatten ``world``, build its BVH on the GPU and keep it resident."""
+    flat = flatten_scene(world)
+    scene = DeviceScene(flat, device=device, max_leaf=max_leaf, method=method)
+    out = ExtractResult(scene.stats())
+    out.scene = scene
+    out.flat = flat
+    return out

@@ -1,0 +1,125 @@
+"""Shared test helpers: scene conversion for the oracle and paired renders."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from oracle import bvh_oracle
+from oracle import oracle as orc
+from roxy_dcc_amd import scenes
+from roxy_dcc_amd.bvh import TriangleBatch
+from roxy_dcc_amd.extract import flatten_scene
+
+
+def material_rows(batch: TriangleBatch):
+    """Per-triangle (type, albedo, roughness, ior, emission, texture) rows."""
+    if batch.mat_idx is not None:
+        pal = batch.mat_palette
+        rows = [(p["type"], tuple(p["albedo"]), p["roughness"], p["ior"], p["emission"], p.get("texture_idx", -1))
+                for p in pal]
+        return [rows[i] for i in batch.mat_idx]
+    return [(int(batch.mat_type[i]), tuple(batch.albedo[i].tolist()), float(batch.roughness[i]),
+             float(batch.ior[i]), float(batch.emission[i]), int(batch.texture_idx[i]))
+            for i in range(batch.triangle_count)]
+
+
+def prims_dict(prims):
+    return {"obj_type": prims.obj_type, "obj_center": prims.center, "obj_radius": prims.radius,
+            "obj_normal": prims.normal, "obj_offset": prims.offset, "obj_extra": prims.extra, "obj_v0": prims.v0,
+            "obj_v1": prims.v1, "obj_v2": prims.v2, "obj_mat_type": prims.mat_type, "obj_albedo": prims.albedo,
+            "obj_roughness": prims.roughness, "obj_ior": prims.ior, "obj_emission": prims.emission}
+
+
+def flat_to_oracle(flat, builder="morton", max_leaf=8):
+    tb = flat.triangles()
+    palette, idx = bvh_oracle.material_palette(material_rows(tb))
+    return orc.OracleScene(tb.v0, tb.v1, tb.v2, tb.n0, tb.n1, tb.n2, idx, palette, prims_dict(flat.prims),
+                           builder=builder, max_leaf=max_leaf)
+
+
+def load(name, seed=None):
+    fn = scenes.CONFIGS[name]
+    world, w, h, settings = fn() if seed is None else fn(seed)
+    return world, w, h, settings
+
+
+def oracle_camera(cam, width, height, aspect=None):
+    return orc.camera(cam.position, cam.forward, cam.right, cam.up, math.tan(math.radians(cam.fov) / 2),
+                      cam.aspect_ratio if aspect is None else aspect, width, height)
+
+
+def camera_rays_np(cam, width, height, jitter=0.5):
+    """Deterministic pinhole rays through pixel centres (SoA, 6 x n)."""
+    fov_tan = math.tan(math.radians(cam.fov) / 2)
+    ys, xs = np.mgrid[0:height, 0:width]
+    u = (xs + jitter) / width * 2 - 1
+    v = 1 - (ys + jitter) / height * 2
+    d = (cam.forward[None, None, :] + cam.right[None, None, :] * (u * cam.aspect_ratio * fov_tan)[..., None]
+         + cam.up[None, None, :] * (v * fov_tan)[..., None])
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    d = d.reshape(-1, 3).astype(np.float32)
+    o = np.broadcast_to(np.asarray(cam.position, np.float32), d.shape)
+    return np.ascontiguousarray(np.concatenate([o.T, d.T], axis=0))
+
+
+def random_rays(flat, n, seed=7):
+    """Origins uniform in the scene box, directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    tb = flat.triangles()
+    pts = [tb.v0, tb.v1, tb.v2]
+    if len(flat.prims):
+        pts.append(flat.prims.center)
+    allp = np.concatenate([p for p in pts if len(p)])
+    lo, hi = allp.min(axis=0), allp.max(axis=0)
+    o = rng.uniform(lo, hi, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.ascontiguousarray(np.concatenate([o.T, d.T], axis=0).astype(np.float32))
+
+
+def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0, spp_per_pass=None,
+                direct_light_mode=None, threads=None, world_tuple=None):
+    """Render the same scene with the GPU library and the fp32 oracle."""
+    import torch
+
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, w0, h0, settings = world_tuple if world_tuple is not None else load(name)
+    W, H = width or w0, height or h0
+    cam = world.active_camera
+    md = settings["max_depth"] if max_depth is None else max_depth
+    mode = settings["direct_light_mode"] if direct_light_mode is None else direct_light_mode
+    mode_id = R.TaichiRenderer._direct_light_mode_to_id(mode)
+    dlmd = settings["direct_light_max_depth"]
+    ex = extract_scene(world)
+    spp = spp_per_pass or samples
+    integ = R.Integrator(ex.scene, W, H, spp)
+    frame = R.DeviceFrame(W, H)
+    camera = R.camera_struct(cam, W, H)
+    params = R.params_struct(md, int(world._use_sky), mode_id, dlmd, 10.0, world._background_color, seed, spp)
+    done = 0
+    while done < samples:
+        n = min(spp, samples - done)
+        integ.render(frame, camera, params, done, n)
+        done += n
+    torch.cuda.synchronize()
+    gpu = {k: getattr(frame, k).cpu().numpy() for k in ("color", "normal", "albedo", "depth", "display")}
+    counts = integ.counts()
+
+    osc = flat_to_oracle(ex.flat)
+    ocam = oracle_camera(cam, W, H)
+    oprm = orc.params(md, int(world._use_sky), mode_id, dlmd, 10.0, world._background_color, seed)
+    acc = orc.Accum(W * H)
+    orc.render(osc, ocam, oprm, 0, samples, acc, threads=threads)
+    return {
+        "gpu_color": gpu["color"].reshape(-1, 3), "cpu_color": acc.color,
+        "gpu_normal": gpu["normal"].reshape(-1, 3), "cpu_normal": acc.normal,
+        "gpu_albedo": gpu["albedo"].reshape(-1, 3), "cpu_albedo": acc.albedo,
+        "gpu_depth": gpu["depth"].reshape(-1), "cpu_depth": acc.depth,
+        "gpu_display": gpu["display"].reshape(-1, 3),
+        "gpu_rays": counts["rays"], "cpu_rays": acc.counts.rays,
+        "gpu_counts": counts, "cpu_counts": acc.counts.as_dict(),
+        "extract": ex, "oracle_scene": osc,
+    }

@@ -1,0 +1,253 @@
+"""Host-side logic and the C ABI surface (no GPU needed)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import bvh_oracle
+from oracle import denoise_oracle
+from roxy_dcc_amd import _native as nat
+from roxy_dcc_amd import scenes
+from roxy_dcc_amd.bvh import GPUBVHBuilder, TriangleBatch
+from roxy_dcc_amd.extract import flatten_scene
+from roxy_dcc_amd.render_stats import RenderStats
+from roxy_dcc_amd.renderer import TaichiRenderer, default_samples_per_pass
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+# ------------------------------------------------------------------ C ABI
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "roxy_hip.h").read_text()
+    declared = set(re.findall(r"^\w[\w\s\*]*?\b(rx_\w+)\s*\(", header, re.M))
+    assert declared == set(nat.SIGNATURES)
+    lib = nat.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.rx_version() == 1
+
+
+def test_morton_shape_closed_form_matches_recursion():
+    for n in list(range(1, 40)) + [69_630, 100_003, 2_000_000]:
+        for leaf in (1, 4, 8):
+            a, b = C.c_int(), C.c_int()
+            assert nat.lib().rx_bvh_morton_shape(n, leaf, C.byref(a), C.byref(b)) == 0
+            assert (a.value, b.value) == bvh_oracle.closed_form_shape(n, leaf), (n, leaf)
+
+
+def test_abi_errors_are_reported_without_touching_the_gpu():
+    lib = nat.lib()
+    a, b = C.c_int(), C.c_int()
+    assert lib.rx_bvh_morton_shape(10, 0, C.byref(a), C.byref(b)) == nat.RX_ERR_INVALID
+    assert "max_leaf" in nat.last_error()
+    ws = C.c_size_t()
+    assert lib.rx_bvh_workspace_size(nat.RX_MAX_TRIANGLES + 1, 8, C.byref(ws)) == nat.RX_ERR_CAPACITY
+    with pytest.raises(AssertionError):
+        nat.check(nat.RX_ERR_CAPACITY)
+    assert lib.rx_render(None, None, None, 0, 1, None, 0, None, None, None, None) == nat.RX_ERR_INVALID
+    sc = nat.RxScene()
+    assert lib.rx_closest_hit(C.byref(sc), None, 5, None, None, None, None, None, None, None) == nat.RX_ERR_INVALID
+    assert lib.rx_denoise(None, None, None, None, 4, 4, 1, 1, 0.1, 0.1, 0.1, 0.1, 1.0, None, None) \
+        == nat.RX_ERR_INVALID
+    with pytest.raises(ValueError):
+        nat.check(lib.rx_bvh_morton_shape(-1, 8, C.byref(a), C.byref(b)))
+
+
+def test_render_workspace_size_grows_with_paths():
+    ws1, ws2 = C.c_size_t(), C.c_size_t()
+    assert nat.lib().rx_render_workspace_size(64, 64, 1, 0, C.byref(ws1)) == 0
+    assert nat.lib().rx_render_workspace_size(64, 64, 4, 0, C.byref(ws2)) == 0
+    assert ws2.value > 3 * ws1.value
+    # 47 four-byte words per path
+    assert ws1.value >= 64 * 64 * 47 * 4
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors have the sizes and field offsets gcc gives the header."""
+    import subprocess
+
+    structs = {"rx_scene": nat.RxScene, "rx_camera": nat.RxCamera, "rx_render_params": nat.RxRenderParams,
+               "rx_accum": nat.RxAccum, "rx_counters": nat.RxCounters, "rx_profile": nat.RxProfile,
+               "rx_bvh_ref": nat.RxBvhRef}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "roxy_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    for cname, size in (("rx_node", 64), ("rx_bvh_header", 32), ("rx_tri", 48), ("rx_tri_shade", 48),
+                        ("rx_prim", 48), ("rx_material", 32), ("rx_light", 32)):
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, (cname, field)
+    assert int(got["rx_node"]) == nat.NODE_BYTES and int(got["rx_tri"]) == nat.TRI_BYTES
+    assert int(got["rx_bvh_header"]) == nat.HEADER_BYTES and int(got["rx_prim"]) == nat.PRIM_BYTES
+    assert int(got["rx_material"]) == nat.MATERIAL_BYTES and int(got["rx_light"]) == nat.LIGHT_BYTES
+    assert int(got["rx_tri_shade"]) == nat.SHADE_BYTES
+
+
+# ------------------------------------------------------------------ TriangleBatch
+def _tri(x, mat_type=0, albedo=None):
+    return {"v0": [x, 0, 0], "v1": [x + 1, 0, 0], "v2": [x, 1, 0], "n0": [0, 0, 1], "n1": [0, 0, 1],
+            "n2": [0, 0, 1], "mat_type": mat_type, "albedo": albedo or [1, 1, 1], "roughness": 0.0, "ior": 1.0,
+            "emission": 0.0}
+
+
+def test_triangle_batch_from_dicts_and_defaults():
+    b = TriangleBatch.from_dicts([_tri(0), _tri(2)])
+    assert b.triangle_count == 2 and b.mat_idx is None
+    assert b.uv0.shape == (2, 2) and b.has_uv.tolist() == [0, 0] and b.texture_idx.tolist() == [-1, -1]
+    assert TriangleBatch.from_dicts([]).triangle_count == 0
+
+
+def test_triangle_batch_validation():
+    with pytest.raises(ValueError, match="same length"):
+        TriangleBatch(np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((3, 3)), np.zeros((2, 3)), np.zeros((2, 3)),
+                      np.zeros((2, 3)), [0, 0], np.zeros((2, 3)), [0, 0], [1, 1], [0, 0])
+    with pytest.raises(ValueError, match="match triangle count"):
+        TriangleBatch(np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3)),
+                      np.zeros((2, 3)), [0], np.zeros((1, 3)), [0], [1], [0])
+
+
+def test_indexed_batch_keeps_compact_palette():
+    # tests/bvh_builder_tests.py:78-108
+    arrays = {"v0": [[0, 0, 0], [2, 0, 0], [4, 0, 0]], "v1": [[1, 0, 0], [3, 0, 0], [5, 0, 0]],
+              "v2": [[0, 1, 0], [2, 1, 0], [4, 1, 0]], "n0": [[0, 0, 1]] * 3, "n1": [[0, 0, 1]] * 3,
+              "n2": [[0, 0, 1]] * 3, "uv0": [[0, 0]] * 3, "uv1": [[1, 0]] * 3, "uv2": [[0, 1]] * 3,
+              "has_uv": [1, 1, 1], "group_idx": [0, 1, 0]}
+    mats = {0: {"mat_type": 0, "albedo": [1, 0, 0], "roughness": 0, "ior": 1, "emission": 0, "texture_idx": 3},
+            1: {"mat_type": 1, "albedo": [0, 1, 0], "roughness": 0.25, "ior": 1, "emission": 0, "texture_idx": -1},
+            2: {"mat_type": 0, "albedo": [0, 0, 1], "roughness": 0, "ior": 1, "emission": 0, "texture_idx": -1}}
+    batch = TriangleBatch.from_indexed_arrays(arrays, mats)
+    assert batch.triangle_count == 3 and len(batch.mat_palette) == 2 and len(batch.mat_type) == 2
+    assert list(batch.mat_idx) == [0, 1, 0] and list(batch.has_uv) == [1, 1, 1]
+    assert batch.mat_palette[0]["texture_idx"] == 3
+
+
+def test_concat_offsets_palettes():
+    a = TriangleBatch.from_indexed_arrays(
+        {"v0": [[0, 0, 0]], "v1": [[1, 0, 0]], "v2": [[0, 1, 0]], "n0": [[0, 0, 1]], "n1": [[0, 0, 1]],
+         "n2": [[0, 0, 1]], "group_idx": [0]}, {0: {"mat_type": 0, "albedo": [1, 0, 0], "roughness": 0, "ior": 1,
+                                                    "emission": 0}})
+    c = TriangleBatch.concat([a, a])
+    assert c.triangle_count == 2 and list(c.mat_idx) == [0, 1] and len(c.mat_palette) == 2
+
+
+def test_builder_palette_dedup_first_appearance():
+    # tests/bvh_builder_tests.py:20-33 (palette sizes), without a GPU build
+    b = GPUBVHBuilder()
+    assert len(set(b._build_material_indices(TriangleBatch.from_dicts([_tri(0), _tri(2)])))) == 1
+    b = GPUBVHBuilder()
+    idx = b._build_material_indices(TriangleBatch.from_dicts([_tri(0), _tri(2, albedo=[0.5, 0.5, 0.5]), _tri(3)]))
+    assert idx.tolist() == [0, 1, 0] and len(b.mat_palette) == 2
+    assert b.mat_palette[1]["albedo"] == [0.5, 0.5, 0.5]
+
+
+def test_builder_longest_axis_ties():
+    b = GPUBVHBuilder()
+    assert b._longest_axis([0, 0, 0], [1, 1, 1]) == 0
+    assert b._longest_axis([0, 0, 0], [1, 2, 2]) == 1
+    assert b._longest_axis([0, 0, 0], [1, 1, 3]) == 2
+
+
+# ------------------------------------------------------------------ renderer front end
+def test_renderer_validation_and_defaults():
+    world, *_ = scenes.cornell()
+    with pytest.raises(ValueError, match="direct_light_mode"):
+        TaichiRenderer(world, None, None, direct_light_mode="bogus")
+    r = TaichiRenderer(world, None, None, max_depth=7, direct_light_max_depth=None)
+    assert r._direct_light_max_depth == 7
+    assert TaichiRenderer._direct_light_mode_to_id("all") == 1 and TaichiRenderer._direct_light_mode_to_id(0) == 0
+    assert repr(r) == "TaichiRenderer(samples=16, max_depth=7, direct_light_mode='one')"
+
+
+def test_default_samples_per_pass():
+    assert default_samples_per_pass(256, 256, 16) == 16
+    assert default_samples_per_pass(1920, 1080, 16) == 1
+    assert default_samples_per_pass(512, 512, 64) == 8
+
+
+def test_render_stats_report():
+    s = RenderStats(4, 2, 16, 16, 5, 1000, 2.0, primitive_count=3, bvh_nodes=15)
+    assert s.rays_per_pixel == 125.0 and s.rays_per_second == 500.0
+    rep = s.format_report()
+    assert "rays cast:       1,000" in rep and "BVH nodes:       15" in rep
+
+
+# ------------------------------------------------------------------ scenes
+def test_scene_generators_have_the_configured_sizes():
+    flat = flatten_scene(scenes.cornell()[0])
+    assert flat.triangle_count == 36 and len(flat.prims) == 0
+    w, W, H, st = scenes.mesh_scene()
+    flat = flatten_scene(w)
+    assert flat.triangle_count == 10 + 20 * 59 * 59 and len(flat.prims.lights()) == 1
+    assert (W, H, st["samples"], st["max_depth"]) == (512, 512, 64, 8)
+    pos, idx = scenes.uv_sphere(50, 41)
+    assert len(idx) == 4000
+    w, W, H, st = scenes.large_scene(n_meshes=3)
+    flat = flatten_scene(w)
+    assert flat.triangle_count == 12_000 and len(flat.prims) == 67 and len(flat.prims.lights()) == 3
+
+
+def test_cornell_walls_follow_the_configuration():
+    flat = flatten_scene(scenes.cornell()[0])
+    tb = flat.triangles()
+    alb = np.asarray([tb.mat_palette[i]["albedo"] for i in tb.mat_idx], np.float32)
+    red = np.all(np.isclose(alb, [0.65, 0.05, 0.05]), axis=1)
+    assert red.sum() == 2 and np.all(tb.v0[red][:, 0] == 0)  # left wall is x = 0
+    em = np.asarray([tb.mat_palette[i]["type"] for i in tb.mat_idx]) == 3
+    assert em.sum() == 2 and np.allclose(tb.v0[em][:, 1], 0.999)
+
+
+# ------------------------------------------------------------------ denoise oracle
+def test_denoise_oracle_reference_cases():
+    # tests/denoise_tests.py:7-107
+    assert denoise_oracle.display(np.array([[[-1.0, 0.25, 4.0]]], np.float32)).tolist() == [[[0.0, 0.5, 1.0]]]
+    p = np.zeros((5, 5, 3), np.float32)
+    p[:, :] = 0.2
+    p[2, 2] = 1.0
+    f = denoise_oracle.denoise(p, radius=1, sigma_color=0.2, amount=0.8)
+    assert f.shape == p.shape and np.isfinite(f).all() and f[2, 2, 0] > f[0, 0, 0]
+    q = np.random.default_rng(1).random((4, 4, 3), dtype=np.float32)
+    assert denoise_oracle.denoise(q, radius=0) is q and denoise_oracle.denoise(q, amount=0.0) is q
+    b = np.zeros((3, 5, 3), np.float32)
+    b[:, 2:] = 1.0
+    plain = denoise_oracle.denoise(b, 1, 10.0, 1.0)
+    nrm = np.zeros((3, 5, 3), np.float32)
+    nrm[:, :2] = [0, 1, 0]
+    nrm[:, 2:] = [0, -1, 0]
+    dep = np.ones((3, 5), np.float32)
+    dep[:, 2:] = 10.0
+    alb = np.zeros((3, 5, 3), np.float32)
+    alb[:, :2] = 0.1
+    alb[:, 2:] = 0.9
+    for kw in ({"normal": nrm, "sigma_normal": 0.1}, {"albedo": alb, "sigma_albedo": 0.05},
+               {"depth": dep, "sigma_depth": 0.1}):
+        g = denoise_oracle.denoise(b, 1, 10.0, 1.0, **kw)
+        assert g[1, 1, 0] < plain[1, 1, 0] * 0.25 and g[1, 2, 0] > plain[1, 2, 0] + 0.20
+    with pytest.raises(ValueError, match="normal guide"):
+        denoise_oracle.denoise(b, normal=np.zeros((3, 4, 3), np.float32))
+
+
+def test_denoise_oracle_fixed_point_and_dilation():
+    u = np.full((9, 11, 3), 0.37, np.float32)
+    np.testing.assert_allclose(denoise_oracle.denoise(u, 2, 0.1, 1.0, dilation=3), u, rtol=1e-6)
+    # a dilated pass equals the dense filter on the sub-lattice when the
+    # image is periodic-free and the taps stay inside
+    img = np.random.default_rng(3).random((40, 40, 3), dtype=np.float32)
+    d2 = denoise_oracle.denoise(img, 1, 0.3, 1.0, dilation=2)
+    sub = img[10:31:2, 10:31:2]
+    dense = denoise_oracle.denoise(sub, 1, 0.3, 1.0)
+    np.testing.assert_allclose(d2[12:29:2, 12:29:2], dense[1:-1, 1:-1], rtol=1e-6)
