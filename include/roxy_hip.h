@@ -300,6 +300,41 @@ int rx_render(const rx_scene* scene, const rx_camera* camera,
               const rx_accum* accum, rx_counters* d_counters,
               rx_profile* profile, void* stream);
 
+/* ---- batches: many scenes in one wavefront ------------------------------
+ * One job = one scene, camera, settings and accumulation target.  A batch
+ * traces a pass of every unfinished job together: each pass renders
+ * min(samples_left, samples_per_pass) samples of every job, in the
+ * job's sample order, so a job's result does not depend on the others in
+ * the batch.  first_sample/samples_left/rays advance on the device, so a
+ * pass can be captured once in a HIP graph and replayed.  No reference
+ * counterpart: render_kernel renders one scene per launch
+ * (rendering/taichi/trace.py:239); this is the batched form of it. */
+typedef struct rx_job {
+    rx_scene scene;
+    rx_camera camera;
+    rx_render_params params;   /* params.samples_per_pass = samples per pass */
+    rx_accum accum;
+    int32_t first_sample;      /* next sample index (advanced per pass) */
+    int32_t samples_left;      /* samples still to render (decremented per pass) */
+    int32_t path_offset;       /* set by the library: first path slot */
+    int32_t pixel_offset;      /* set by the library: first pixel slot */
+    unsigned long long rays;   /* rays cast for this job (accumulated) */
+} rx_job;
+
+typedef struct rx_batch rx_batch;
+
+int rx_batch_workspace_size(const rx_job* jobs, int n_jobs, size_t* bytes);
+/* Copies the job records into the workspace and prepares the passes.
+ * use_graph != 0 captures one pass in a HIP graph on the first run and
+ * replays it.  The workspace and every pointer in the jobs must stay
+ * valid until rx_batch_destroy. */
+int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace, size_t workspace_bytes,
+                    int use_graph, rx_batch** out);
+int rx_batch_run(rx_batch* batch, int passes, rx_counters* d_counters, rx_profile* profile, void* stream);
+/* Copies the job records (progress and ray counts) back; synchronises `stream`. */
+int rx_batch_read_jobs(rx_batch* batch, rx_job* jobs_out, void* stream);
+int rx_batch_destroy(rx_batch* batch);
+
 /* ---- denoise: rendering/denoise.py:23-108 -------------------------------
  * Guided bilateral filter over a (2*radius+1)^2 window whose taps are
  * `dilation` pixels apart (dilation 1 is the reference filter), reflect

@@ -84,6 +84,12 @@ class RxProfile(C.Structure):
     ]
 
 
+class RxJob(C.Structure):
+    _fields_ = [("scene", RxScene), ("camera", RxCamera), ("params", RxRenderParams), ("accum", RxAccum),
+                ("first_sample", i32), ("samples_left", i32), ("path_offset", i32), ("pixel_offset", i32),
+                ("rays", C.c_ulonglong)]
+
+
 class RxBvhRef(C.Structure):
     _fields_ = [("aabb_min", vp), ("aabb_max", vp), ("left", vp), ("right", vp), ("tri_start", vp),
                 ("tri_count", vp)]
@@ -104,6 +110,11 @@ SIGNATURES = {
     "rx_render_workspace_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "rx_render": (C.c_int, [C.POINTER(RxScene), C.POINTER(RxCamera), C.POINTER(RxRenderParams), C.c_int, C.c_int,
                             vp, C.c_size_t, C.POINTER(RxAccum), vp, C.POINTER(RxProfile), vp]),
+    "rx_batch_workspace_size": (C.c_int, [C.POINTER(RxJob), C.c_int, C.POINTER(C.c_size_t)]),
+    "rx_batch_create": (C.c_int, [C.POINTER(RxJob), C.c_int, vp, C.c_size_t, C.c_int, C.POINTER(vp)]),
+    "rx_batch_run": (C.c_int, [vp, C.c_int, vp, C.POINTER(RxProfile), vp]),
+    "rx_batch_read_jobs": (C.c_int, [vp, C.POINTER(RxJob), vp]),
+    "rx_batch_destroy": (C.c_int, [vp]),
     "rx_denoise": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                              C.c_float, C.c_float, vp, vp]),
     "rx_display": (C.c_int, [vp, C.c_int, vp, vp]),

@@ -1,19 +1,28 @@
 // rx_render.hip — wavefront path tracer reproducing render_kernel's
-// estimator (rendering/taichi/trace.py:139-283).
+// estimator (rendering/taichi/trace.py:139-283), for one scene or a batch
+// of scenes.
 //
-// A pass traces `ns` samples of every pixel at once, as P = ns * W * H
-// paths (path p = sample (p / (W*H)), pixel (p % (W*H))).  Per bounce:
+// A pass traces, for every job of the batch, ns = min(samples_left,
+// samples_per_pass) samples of each of its pixels: job j owns the path
+// slots [path_offset, path_offset + W*H*samples_per_pass), slot
+// path_offset + s*W*H + pixel for its sample s.  Per bounce:
 //   extend   closest hit for every queued ray (SoA queue in HBM)
 //   shade    miss/emission/material, next-event request, BSDF sample,
 //            Russian roulette; survivors and next-event requests are
-//            compacted into the next queues with one atomic per wave
+//            appended to the next queues with one atomic per wave
 //            (ballot + popcount)
 //   connect  sphere-light sampling and any-hit shadow rays for each
 //            next-event request, added to the path's colour
-// then accumulate folds the pass's samples into the running means in
-// sample order.  Randomness is Philox4x32-10 keyed by the seed and
-// counted by (pixel, sample, bounce, draw), so a pixel's samples do not
-// depend on the pass size, the queue order or what else is in flight.
+// then accumulate folds each job's samples into its running means in
+// sample order, and advance moves every job's sample window.  All
+// per-pass state lives on the device, so a pass can be captured in a HIP
+// graph and replayed.  Randomness is Philox4x32-10 keyed by the job's seed
+// and counted by (pixel, sample, bounce, draw): a pixel's samples do not
+// depend on the pass size, the queue order or the other jobs.
+#include <stdlib.h>
+
+#include <new>
+
 #include "rx_launch.h"
 
 namespace {
@@ -21,12 +30,20 @@ namespace {
 using namespace rx;
 
 constexpr int BS = 256;
+constexpr int MAX_JOBS = 4096;
 
 // Path state and queues, SoA over the workspace.
 struct ws_view {
+    rx_job* jobs;
+    int32_t* path_off;   // n_jobs + 1
+    int32_t* pix_off;    // n_jobs + 1
+    int n_jobs;
+    long long P;         // path slots
+    long long NP;        // pixel slots
     // per path
     float *thr_x, *thr_y, *thr_z, *col_x, *col_y, *col_z, *ior;
     float *fn_x, *fn_y, *fn_z, *fa_x, *fa_y, *fa_z, *fd;
+    int32_t *p_job, *p_rays;
     // two ray queues
     int32_t* q_path[2];
     float *q_ox[2], *q_oy[2], *q_oz[2], *q_dx[2], *q_dy[2], *q_dz[2];
@@ -41,10 +58,16 @@ struct ws_view {
     int32_t* counts;
 };
 
-void carve(rx_arena& ar, long long P, ws_view& w) {
+void carve(rx_arena& ar, int n_jobs, long long P, ws_view& w) {
+    w.jobs = ar.take<rx_job>(n_jobs);
+    w.path_off = ar.take<int32_t>(n_jobs + 1);
+    w.pix_off = ar.take<int32_t>(n_jobs + 1);
+    w.counts = ar.take<int32_t>(8);
     float** pf[] = {&w.thr_x, &w.thr_y, &w.thr_z, &w.col_x, &w.col_y, &w.col_z, &w.ior,
                     &w.fn_x,  &w.fn_y,  &w.fn_z,  &w.fa_x,  &w.fa_y,  &w.fa_z,  &w.fd};
     for (float** f : pf) *f = ar.take<float>(P);
+    w.p_job = ar.take<int32_t>(P);
+    w.p_rays = ar.take<int32_t>(P);
     for (int b = 0; b < 2; ++b) {
         w.q_path[b] = ar.take<int32_t>(P);
         w.q_ox[b] = ar.take<float>(P);
@@ -63,15 +86,29 @@ void carve(rx_arena& ar, long long P, ws_view& w) {
     float** pe[] = {&w.e_px, &w.e_py, &w.e_pz, &w.e_nx, &w.e_ny, &w.e_nz, &w.e_ax,
                     &w.e_ay, &w.e_az, &w.e_tx, &w.e_ty, &w.e_tz, &w.e_scale};
     for (float** f : pe) *f = ar.take<float>(P);
-    w.counts = ar.take<int32_t>(8);
+    w.n_jobs = n_jobs;
+    w.P = P;
 }
 
-struct pass_info {
-    int npix;       // W * H
-    int first;      // first sample index of the pass
-    int ns;         // samples in the pass
-    int depth;      // bounce index of the current launch
-};
+// Job owning slot `x` of a prefix-offset table (largest j with off[j] <= x).
+__device__ __forceinline__ int owner(const int32_t* __restrict__ off, int n, long long x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((long long)off[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int job_of_path(const ws_view& w, int p) { return w.n_jobs == 1 ? 0 : w.p_job[p]; }
+
+__device__ __forceinline__ int pass_samples(const rx_job& j) {
+    int spp = j.params.samples_per_pass;
+    return j.samples_left < spp ? j.samples_left : spp;
+}
 
 __device__ __forceinline__ u4 rng(const rx_render_params& prm, int pix, int frame, int depth, uint32_t draw) {
     return philox4x32_10(u4{(uint32_t)pix, (uint32_t)frame, (uint32_t)depth, draw}, prm.seed_lo, prm.seed_hi);
@@ -91,53 +128,82 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     return base + __popcll(below);
 }
 
+// ---------------------------------------------------------------- pass control
+__global__ void k_set_job(rx_job* jobs, rx_job j, int32_t* path_off, int32_t* pix_off, int32_t paths, int32_t pix) {
+    jobs[0] = j;
+    path_off[0] = 0;
+    path_off[1] = paths;
+    pix_off[0] = 0;
+    pix_off[1] = pix;
+}
+
+__global__ void k_pass_begin(int32_t* counts) {
+    if (threadIdx.x < 8) counts[threadIdx.x] = 0;
+}
+
+// Moves every job's sample window after a pass.
+__global__ void k_advance(rx_job* jobs, int n_jobs) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_jobs) return;
+    int ns = pass_samples(jobs[j]);
+    jobs[j].first_sample += ns;
+    jobs[j].samples_left -= ns;
+}
+
 // ---------------------------------------------------------------- generate
-__global__ __launch_bounds__(BS) void k_generate(rx_camera cam, pass_info pi, ws_view w) {
-    long long P = (long long)pi.npix * pi.ns;
-    int stride = gridDim.x * BS;
-    for (long long base = (long long)xcd_block() * BS; base < P; base += stride) {
+__global__ __launch_bounds__(BS) void k_generate(ws_view w) {
+    const int stride = gridDim.x * BS;
+    for (long long base = (long long)xcd_block() * BS; base < w.P; base += stride) {
         long long p = base + threadIdx.x;
-        if (p < P) {
-            int s = (int)(p / pi.npix), pix = (int)(p % pi.npix);
-            int x = pix % cam.width, y = pix / cam.width;
-            f3 rd = camera_dir(cam, x, y, pi.first + s);
-            w.q_path[0][p] = (int32_t)p;
-            w.q_ox[0][p] = cam.pos[0];
-            w.q_oy[0][p] = cam.pos[1];
-            w.q_oz[0][p] = cam.pos[2];
-            w.q_dx[0][p] = rd.x;
-            w.q_dy[0][p] = rd.y;
-            w.q_dz[0][p] = rd.z;
-            w.thr_x[p] = 1.0f;
-            w.thr_y[p] = 1.0f;
-            w.thr_z[p] = 1.0f;
-            w.col_x[p] = 0.0f;
-            w.col_y[p] = 0.0f;
-            w.col_z[p] = 0.0f;
-            w.ior[p] = 1.0f;
-            w.fn_x[p] = 0.0f;
-            w.fn_y[p] = 0.0f;
-            w.fn_z[p] = 0.0f;
-            w.fa_x[p] = 0.0f;
-            w.fa_y[p] = 0.0f;
-            w.fa_z[p] = 0.0f;
-            w.fd[p] = 0.0f;
+        bool active = false;
+        int jx = 0;
+        f3 org = mk(0.0f, 0.0f, 0.0f), rd = org;
+        if (p < w.P) {
+            jx = w.n_jobs == 1 ? 0 : owner(w.path_off, w.n_jobs, p);
+            const rx_job& J = w.jobs[jx];
+            int npix = J.camera.width * J.camera.height;
+            long long local = p - w.path_off[jx];
+            int s = (int)(local / npix), pix = (int)(local % npix);
+            if (s < pass_samples(J)) {
+                active = true;
+                rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
+                org = ld3(J.camera.pos);
+                w.thr_x[p] = 1.0f;
+                w.thr_y[p] = 1.0f;
+                w.thr_z[p] = 1.0f;
+                w.col_x[p] = 0.0f;
+                w.col_y[p] = 0.0f;
+                w.col_z[p] = 0.0f;
+                w.ior[p] = 1.0f;
+                w.fn_x[p] = 0.0f;
+                w.fn_y[p] = 0.0f;
+                w.fn_z[p] = 0.0f;
+                w.fa_x[p] = 0.0f;
+                w.fa_y[p] = 0.0f;
+                w.fa_z[p] = 0.0f;
+                w.fd[p] = 0.0f;
+                w.p_job[p] = jx;
+                w.p_rays[p] = 0;
+            }
         }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        w.counts[0] = (int32_t)P;
-        w.counts[1] = 0;
-        w.counts[2] = 0;
+        int slot = wave_append(&w.counts[0], active);
+        if (active) {
+            w.q_path[0][slot] = (int32_t)p;
+            w.q_ox[0][slot] = org.x;
+            w.q_oy[0][slot] = org.y;
+            w.q_oz[0][slot] = org.z;
+            w.q_dx[0][slot] = rd.x;
+            w.q_dy[0][slot] = rd.y;
+            w.q_dz[0][slot] = rd.z;
+        }
     }
 }
 
 // ---------------------------------------------------------------- extend
 template <int STACK, int SBS, bool COUNT>
-__global__ __launch_bounds__(SBS) void k_extend(rx_scene sc, ws_view w, int qi, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK * SBS];
     lane_stack<SBS> stk{stack_mem + threadIdx.x};
-    rx_bvh_header hdr;
-    if (sc.n_tris > 0) hdr = *sc.header;
     const int n = w.counts[qi];
     // the queue this bounce's shade fills and the next-event queue start empty
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -150,6 +216,10 @@ __global__ __launch_bounds__(SBS) void k_extend(rx_scene sc, ws_view w, int qi, 
     for (int base = xcd_block() * SBS; base < n; base += stride) {
         int i = base + threadIdx.x;
         if (i < n) {
+            int p = w.q_path[qi][i];
+            const rx_scene& sc = w.jobs[job_of_path(w, p)].scene;
+            rx_bvh_header hdr;
+            if (sc.n_tris > 0) hdr = *sc.header;
             f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
             f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
             hit_rec h = scene_closest<SBS, COUNT>(sc, hdr, ro, rd, stk, cnt);
@@ -158,6 +228,7 @@ __global__ __launch_bounds__(SBS) void k_extend(rx_scene sc, ws_view w, int qi, 
             w.h_kind[i] = h.kind;
             w.h_u[i] = h.u;
             w.h_v[i] = h.v;
+            w.p_rays[p] += 1;
             rays++;
         }
     }
@@ -221,12 +292,9 @@ __device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
 }
 
 // ---------------------------------------------------------------- shade
-__global__ __launch_bounds__(BS) void k_shade(rx_scene sc, rx_render_params prm, pass_info pi, ws_view w, int qi) {
+__global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
     const int n = w.counts[qi];
     const int qo = 1 - qi;
-    const int depth = pi.depth;
-    const bool last = depth + 1 >= prm.max_depth;
-    const bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
     const int stride = gridDim.x * BS;
     for (int base = xcd_block() * BS; base < n; base += stride) {
         int i = base + threadIdx.x;
@@ -237,7 +305,15 @@ __global__ __launch_bounds__(BS) void k_shade(rx_scene sc, rx_render_params prm,
         float scale = 1.0f;
         if (valid) {
             p = w.q_path[qi][i];
-            int pix = p % pi.npix, frame = pi.first + p / pi.npix;
+            int jx = job_of_path(w, p);
+            const rx_job& J = w.jobs[jx];
+            const rx_scene& sc = J.scene;
+            const rx_render_params& prm = J.params;
+            int npix = J.camera.width * J.camera.height;
+            int local = p - w.path_off[jx];
+            int pix = local % npix, frame = J.first_sample + local / npix;
+            bool last = depth + 1 >= prm.max_depth;
+            bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
             f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
             f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
             float t = w.h_t[i];
@@ -410,15 +486,10 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // next-event request, then colour += (throughput * direct) * scale
 // (scale = 1 for diffuse, roughness for glossy, trace.py:199-212).
 template <int STACK, int SBS, bool COUNT>
-__global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params prm, pass_info pi, ws_view w,
-                                                 rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK * SBS];
     lane_stack<SBS> stk{stack_mem + threadIdx.x};
-    rx_bvh_header hdr;
-    if (sc.n_tris > 0) hdr = *sc.header;
     const int n = w.counts[2];
-    const int L = sc.n_lights;
-    const bool all = prm.direct_light_mode == 1;
     trav_count cnt{0, 0, 0};
     unsigned long long shadow = 0;
     const int stride = gridDim.x * SBS;
@@ -426,19 +497,30 @@ __global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params p
         int i = base + threadIdx.x;
         if (i < n) {
             int p = w.e_path[i];
-            int pix = p % pi.npix, frame = pi.first + p / pi.npix;
+            int jx = job_of_path(w, p);
+            const rx_job& J = w.jobs[jx];
+            const rx_scene& sc = J.scene;
+            const rx_render_params& prm = J.params;
+            rx_bvh_header hdr;
+            if (sc.n_tris > 0) hdr = *sc.header;
+            int npix = J.camera.width * J.camera.height;
+            int local = p - w.path_off[jx];
+            int pix = local % npix, frame = J.first_sample + local / npix;
+            const int L = sc.n_lights;
+            const bool all = prm.direct_light_mode == 1;
             f3 hp = mk(w.e_px[i], w.e_py[i], w.e_pz[i]);
             f3 nn = mk(w.e_nx[i], w.e_ny[i], w.e_nz[i]);
             f3 alb = mk(w.e_ax[i], w.e_ay[i], w.e_az[i]);
-            u4 r0 = rng(prm, pix, frame, pi.depth, 0u);
+            u4 r0 = rng(prm, pix, frame, depth, 0u);
             int sel = (int)(u01(r0.x) * (float)L);
             if (sel > L - 1) sel = L - 1;
+            int l0 = all ? 0 : sel, l1 = all ? L : sel + 1;
             f3 direct = mk(0.0f, 0.0f, 0.0f);
-            for (int li = 0; li < L; ++li) {
-                if (!all && li != sel) continue;
+            int cast = 0;
+            for (int li = l0; li < l1; ++li) {
                 float xi1, xi2;
                 if (all) {
-                    u4 rl = rng(prm, pix, frame, pi.depth, 2u + (uint32_t)li);
+                    u4 rl = rng(prm, pix, frame, depth, 2u + (uint32_t)li);
                     xi1 = u01(rl.x);
                     xi2 = u01(rl.y);
                 } else {
@@ -454,7 +536,7 @@ __global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params p
                     float max_t = t_light * 0.999f;
                     if (max_t > 0.001f) {
                         f3 so = hp + nn * 1e-4f;
-                        shadow++;
+                        cast++;
                         if (!scene_occluded<SBS, COUNT>(sc, hdr, so, ld, max_t, stk, cnt)) {
                             float wgt = all ? 1.0f : (float)L;
                             direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
@@ -467,6 +549,8 @@ __global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params p
             w.col_x[p] = w.col_x[p] + add.x;
             w.col_y[p] = w.col_y[p] + add.y;
             w.col_z[p] = w.col_z[p] + add.z;
+            w.p_rays[p] += cast;
+            shadow += cast;
         }
     }
     wave_add(&ctr->rays, shadow);
@@ -480,53 +564,76 @@ __global__ __launch_bounds__(SBS) void k_connect(rx_scene sc, rx_render_params p
 
 // ---------------------------------------------------------------- accumulate
 // rendering/taichi/trace.py:262-283: clamp, running mean of colour and the
-// first-hit guides in sample order, display = sqrt(min(acc, 1)).
-__global__ __launch_bounds__(BS) void k_accumulate(rx_render_params prm, pass_info pi, ws_view w, rx_accum acc) {
+// first-hit guides in sample order, display = sqrt(min(acc, 1)); the
+// pass's rays are added to the job's count.
+__global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
     const int stride = gridDim.x * BS;
-    for (int base = xcd_block() * BS; base < pi.npix; base += stride) {
-        int pix = base + threadIdx.x;
-        if (pix >= pi.npix) continue;
-        f3 c = ld3(acc.color + 3 * pix), nn = ld3(acc.normal + 3 * pix), aa = ld3(acc.albedo + 3 * pix);
-        float dd = acc.depth[pix];
-        for (int s = 0; s < pi.ns; ++s) {
-            long long p = (long long)s * pi.npix + pix;
-            int frame = pi.first + s;
-            f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
-            if (prm.sample_clamp > 0.0f) {
-                float lim = prm.sample_clamp;
-                smp = mk(fminf(fmaxf(smp.x, 0.0f), lim), fminf(fmaxf(smp.y, 0.0f), lim),
-                         fminf(fmaxf(smp.z, 0.0f), lim));
-            }
-            f3 ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
-            f3 as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
-            float ds = w.fd[p];
-            if (frame == 0) {
-                c = smp;
-                nn = ns_;
-                aa = as_;
-                dd = ds;
-            } else {
-                float f = (float)frame, f1 = (float)(frame + 1);
-                c = (c * f + smp) / f1;
-                nn = (nn * f + ns_) / f1;
-                aa = (aa * f + as_) / f1;
-                dd = (dd * f + ds) / f1;
+    for (long long base = (long long)xcd_block() * BS; base < w.NP; base += stride) {
+        long long q = base + threadIdx.x;
+        int jx = 0;
+        unsigned long long rays = 0;
+        if (q < w.NP) {
+            jx = w.n_jobs == 1 ? 0 : owner(w.pix_off, w.n_jobs, q);
+            const rx_job& J = w.jobs[jx];
+            const rx_render_params& prm = J.params;
+            const rx_accum& acc = J.accum;
+            int npix = J.camera.width * J.camera.height;
+            int pix = (int)(q - w.pix_off[jx]);
+            int ns = pass_samples(J);
+            if (ns > 0) {
+                f3 c = ld3(acc.color + 3 * pix), nn = ld3(acc.normal + 3 * pix), aa = ld3(acc.albedo + 3 * pix);
+                float dd = acc.depth[pix];
+                for (int s = 0; s < ns; ++s) {
+                    long long p = (long long)w.path_off[jx] + (long long)s * npix + pix;
+                    int frame = J.first_sample + s;
+                    f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+                    if (prm.sample_clamp > 0.0f) {
+                        float lim = prm.sample_clamp;
+                        smp = mk(fminf(fmaxf(smp.x, 0.0f), lim), fminf(fmaxf(smp.y, 0.0f), lim),
+                                 fminf(fmaxf(smp.z, 0.0f), lim));
+                    }
+                    f3 ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
+                    f3 as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
+                    float ds = w.fd[p];
+                    if (frame == 0) {
+                        c = smp;
+                        nn = ns_;
+                        aa = as_;
+                        dd = ds;
+                    } else {
+                        float f = (float)frame, f1 = (float)(frame + 1);
+                        c = (c * f + smp) / f1;
+                        nn = (nn * f + ns_) / f1;
+                        aa = (aa * f + as_) / f1;
+                        dd = (dd * f + ds) / f1;
+                    }
+                    rays += (unsigned long long)w.p_rays[p];
+                }
+                acc.color[3 * pix] = c.x;
+                acc.color[3 * pix + 1] = c.y;
+                acc.color[3 * pix + 2] = c.z;
+                acc.normal[3 * pix] = nn.x;
+                acc.normal[3 * pix + 1] = nn.y;
+                acc.normal[3 * pix + 2] = nn.z;
+                acc.albedo[3 * pix] = aa.x;
+                acc.albedo[3 * pix + 1] = aa.y;
+                acc.albedo[3 * pix + 2] = aa.z;
+                acc.depth[pix] = dd;
+                if (acc.display) {
+                    acc.display[3 * pix] = sqrtf(fminf(c.x, 1.0f));
+                    acc.display[3 * pix + 1] = sqrtf(fminf(c.y, 1.0f));
+                    acc.display[3 * pix + 2] = sqrtf(fminf(c.z, 1.0f));
+                }
             }
         }
-        acc.color[3 * pix] = c.x;
-        acc.color[3 * pix + 1] = c.y;
-        acc.color[3 * pix + 2] = c.z;
-        acc.normal[3 * pix] = nn.x;
-        acc.normal[3 * pix + 1] = nn.y;
-        acc.normal[3 * pix + 2] = nn.z;
-        acc.albedo[3 * pix] = aa.x;
-        acc.albedo[3 * pix + 1] = aa.y;
-        acc.albedo[3 * pix + 2] = aa.z;
-        acc.depth[pix] = dd;
-        if (acc.display) {
-            acc.display[3 * pix] = sqrtf(fminf(c.x, 1.0f));
-            acc.display[3 * pix + 1] = sqrtf(fminf(c.y, 1.0f));
-            acc.display[3 * pix + 2] = sqrtf(fminf(c.z, 1.0f));
+        // per-job ray totals: one atomic per wave when the wave is one job
+        int j0 = __shfl(jx, 0, 64);
+        bool uniform = __ballot(jx != j0) == 0ull;
+        if (uniform) {
+            unsigned long long s = wave_sum_u64(rays);
+            if ((threadIdx.x & 63) == 0 && s) atomicAdd(&w.jobs[j0].rays, s);
+        } else if (rays) {
+            atomicAdd(&w.jobs[jx].rays, rays);
         }
     }
 }
@@ -541,7 +648,7 @@ struct stage_timer {
     int n = 0;
     int begin(int st) {
         if (!on) return RX_OK;
-        if (n + 2 > MAXE) return rx::set_error(RX_ERR_INVALID, "too many profiled launches in one call");
+        if (n + 2 > MAXE) return rx::set_error(RX_ERR_INVALID, "too many profiled launches in one pass");
         RX_TRY_HIP(hipEventCreate(&ev[n]));
         RX_TRY_HIP(hipEventCreate(&ev[n + 1]));
         stage[n / 2] = st;
@@ -594,35 +701,129 @@ struct stage_timer {
         if (rx_s_) return rx_s_;          \
     } while (0)
 
+// Host-side description of the launches of one pass.
+struct pass_plan {
+    int max_depth;           // bounces (max over jobs)
+    int nee_depth;           // connect launched for depth < nee_depth
+    bool deep;               // stack class
+    bool count;              // traversal counters
+};
+
 template <int STACK, int SBS>
-int run_pass(const rx_scene& sc, const rx_camera& cam, const rx_render_params& prm, int first, int ns, ws_view& w,
-             const rx_accum& acc, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
-    pass_info pi{cam.width * cam.height, first, ns, 0};
-    long long P = (long long)pi.npix * ns;
-    bool count = (prm.flags & RX_FLAG_COUNT_TRAFFIC) != 0;
-    RX_STAGE(tm, 0, (k_generate<<<grid_for(P, BS, 16), BS, 0, s>>>(cam, pi, w)));
-    int gt = grid_for(P, SBS, SBS == 256 ? 16 : 32);
-    int gs = grid_for(P, BS, 16);
-    for (int d = 0; d < prm.max_depth; ++d) {
+int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+    int gp = grid_for(w.P, BS, 16);
+    int gt = grid_for(w.P, SBS, SBS == 256 ? 16 : 32);
+    RX_STAGE(tm, 0, (k_pass_begin<<<1, 64, 0, s>>>(w.counts)));
+    RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
+    for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
-        pi.depth = d;
-        if (count)
-            RX_STAGE(tm, 1, (k_extend<STACK, SBS, true><<<gt, SBS, 0, s>>>(sc, w, qi, ctr)));
+        if (pl.count)
+            RX_STAGE(tm, 1, (k_extend<STACK, SBS, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
         else
-            RX_STAGE(tm, 1, (k_extend<STACK, SBS, false><<<gt, SBS, 0, s>>>(sc, w, qi, ctr)));
-        RX_STAGE(tm, 2, (k_shade<<<gs, BS, 0, s>>>(sc, prm, pi, w, qi)));
-        if (d < prm.direct_light_max_depth && sc.n_lights > 0) {
-            if (count)
-                RX_STAGE(tm, 3, (k_connect<STACK, SBS, true><<<gt, SBS, 0, s>>>(sc, prm, pi, w, ctr)));
+            RX_STAGE(tm, 1, (k_extend<STACK, SBS, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+        RX_STAGE(tm, 2, (k_shade<<<gp, BS, 0, s>>>(w, qi, d)));
+        if (d < pl.nee_depth) {
+            if (pl.count)
+                RX_STAGE(tm, 3, (k_connect<STACK, SBS, true><<<gt, SBS, 0, s>>>(w, d, ctr)));
             else
-                RX_STAGE(tm, 3, (k_connect<STACK, SBS, false><<<gt, SBS, 0, s>>>(sc, prm, pi, w, ctr)));
+                RX_STAGE(tm, 3, (k_connect<STACK, SBS, false><<<gt, SBS, 0, s>>>(w, d, ctr)));
         }
     }
-    RX_STAGE(tm, 4, (k_accumulate<<<grid_for(pi.npix, BS, 16), BS, 0, s>>>(prm, pi, w, acc)));
+    RX_STAGE(tm, 4, (k_accumulate<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
+    RX_STAGE(tm, 4, (k_advance<<<(w.n_jobs + 255) / 256, 256, 0, s>>>(w.jobs, w.n_jobs)));
+    return RX_OK;
+}
+
+int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+    if (pl.deep) return enqueue_pass<STACK_DEEP, BS_DEEP>(w, pl, ctr, tm, s);
+    return enqueue_pass<STACK_SHALLOW, BS_SHALLOW>(w, pl, ctr, tm, s);
+}
+
+int check_job(const rx_job& j, int idx) {
+    const rx_scene& sc = j.scene;
+    const rx_render_params& p = j.params;
+    RX_REQUIRE(j.camera.width > 0 && j.camera.height > 0, RX_ERR_INVALID, "job %d: camera size must be positive",
+               idx);
+    RX_REQUIRE(j.first_sample >= 0 && j.samples_left >= 0, RX_ERR_INVALID, "job %d: negative sample range", idx);
+    RX_REQUIRE(p.samples_per_pass > 0, RX_ERR_INVALID, "job %d: samples_per_pass must be positive", idx);
+    RX_REQUIRE(p.max_depth >= 0, RX_ERR_INVALID, "job %d: max_depth must be >= 0", idx);
+    RX_REQUIRE(p.direct_light_mode == 0 || p.direct_light_mode == 1, RX_ERR_INVALID,
+               "direct_light_mode must be 'one' or 'all'");
+    RX_REQUIRE(j.accum.color && j.accum.normal && j.accum.albedo && j.accum.depth, RX_ERR_INVALID,
+               "job %d: accumulation buffers must be given", idx);
+    RX_REQUIRE(sc.n_prims >= 0 && sc.n_tris >= 0 && sc.n_lights >= 0, RX_ERR_INVALID, "job %d: negative scene size",
+               idx);
+    RX_REQUIRE(sc.n_prims == 0 || (sc.prims && sc.prim_materials), RX_ERR_INVALID,
+               "job %d: scene has primitives but no primitive arrays", idx);
+    RX_REQUIRE(sc.n_lights == 0 || sc.lights, RX_ERR_INVALID, "job %d: scene has lights but lights is NULL", idx);
+    if (sc.n_tris > 0) {
+        RX_REQUIRE(sc.nodes && sc.header && sc.tris && sc.tri_shade && sc.materials, RX_ERR_INVALID,
+                   "job %d: scene has triangles but no BVH/shading arrays", idx);
+        RX_REQUIRE(sc.bvh_depth >= 0 && sc.bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY, "BVH depth %d exceeds %d",
+                   sc.bvh_depth, RX_MAX_BVH_DEPTH);
+    }
+    return RX_OK;
+}
+
+// Offsets, totals and the launch plan of a job list.
+int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan* pl, int32_t* path_off,
+              int32_t* pix_off) {
+    RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
+    long long p = 0, q = 0;
+    pass_plan plan{0, 0, false, false};
+    for (int i = 0; i < n; ++i) {
+        int st = check_job(jobs[i], i);
+        if (st) return st;
+        const rx_job& j = jobs[i];
+        long long npix = (long long)j.camera.width * j.camera.height;
+        if (path_off) path_off[i] = (int32_t)p;
+        if (pix_off) pix_off[i] = (int32_t)q;
+        p += npix * j.params.samples_per_pass;
+        q += npix;
+        if (j.params.max_depth > plan.max_depth) plan.max_depth = j.params.max_depth;
+        if (j.scene.n_lights > 0 && j.params.direct_light_max_depth > plan.nee_depth)
+            plan.nee_depth = j.params.direct_light_max_depth;
+        if (j.scene.n_tris > 0 && use_deep_stack(j.scene.bvh_depth)) plan.deep = true;
+        if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
+    }
+    RX_REQUIRE(p <= 0x7fffffffLL, RX_ERR_CAPACITY, "batch of %lld paths exceeds 2^31-1", p);
+    if (path_off) path_off[n] = (int32_t)p;
+    if (pix_off) pix_off[n] = (int32_t)q;
+    *P = p;
+    *NP = q;
+    *pl = plan;
+    return RX_OK;
+}
+
+size_t workspace_bytes_for(int n_jobs, long long P) {
+    rx_arena ar{nullptr, 0, 0};
+    ws_view w;
+    carve(ar, n_jobs, P, w);
+    return ar.off + 256;
+}
+
+int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, ws_view& w) {
+    size_t need = workspace_bytes_for(n_jobs, P);
+    RX_REQUIRE(d_workspace && bytes >= need, RX_ERR_WORKSPACE, "render workspace too small: need %zu bytes, have %zu",
+               need, bytes);
+    rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, 0};
+    ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
+    carve(ar, n_jobs, P, w);
+    w.NP = NP;
     return RX_OK;
 }
 
 }  // namespace
+
+struct rx_batch {
+    ws_view w;
+    pass_plan plan;
+    int n_jobs;
+    bool use_graph;
+    hipGraphExec_t exec;
+    bool captured;
+    rx_counters* graph_ctr;
+};
 
 extern "C" int rx_render_workspace_size(int width, int height, int samples_per_pass, int n_lights, size_t* bytes) {
     (void)n_lights;
@@ -630,11 +831,8 @@ extern "C" int rx_render_workspace_size(int width, int height, int samples_per_p
     RX_REQUIRE(width > 0 && height > 0 && samples_per_pass > 0, RX_ERR_INVALID,
                "width, height and samples_per_pass must be positive");
     long long P = (long long)width * height * samples_per_pass;
-    RX_REQUIRE(P <= (1LL << 31) - 1, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
-    rx_arena ar{nullptr, 0, 0};
-    ws_view w;
-    carve(ar, P, w);
-    *bytes = ar.off + 256;
+    RX_REQUIRE(P <= 0x7fffffffLL, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
+    *bytes = workspace_bytes_for(1, P);
     return RX_OK;
 }
 
@@ -642,58 +840,157 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
                          int first_sample, int n_samples, void* d_workspace, size_t workspace_bytes,
                          const rx_accum* accum, rx_counters* d_counters, rx_profile* profile, void* stream) {
     RX_REQUIRE(scene && camera && params && accum && d_counters, RX_ERR_INVALID, "NULL argument");
-    RX_REQUIRE(camera->width > 0 && camera->height > 0, RX_ERR_INVALID, "camera size must be positive");
-    RX_REQUIRE(first_sample >= 0 && n_samples >= 0, RX_ERR_INVALID, "sample range must be non-negative");
-    RX_REQUIRE(params->samples_per_pass > 0, RX_ERR_INVALID, "samples_per_pass must be positive");
-    RX_REQUIRE(params->max_depth >= 0, RX_ERR_INVALID, "max_depth must be >= 0");
-    RX_REQUIRE(params->direct_light_mode == 0 || params->direct_light_mode == 1, RX_ERR_INVALID,
-               "direct_light_mode must be 'one' or 'all'");
-    RX_REQUIRE(accum->color && accum->normal && accum->albedo && accum->depth, RX_ERR_INVALID,
-               "accumulation buffers must be given");
-    RX_REQUIRE(scene->n_prims >= 0 && scene->n_tris >= 0 && scene->n_lights >= 0, RX_ERR_INVALID,
-               "negative scene size");
-    RX_REQUIRE(scene->n_prims == 0 || (scene->prims && scene->prim_materials), RX_ERR_INVALID,
-               "scene has primitives but no primitive arrays");
-    RX_REQUIRE(scene->n_lights == 0 || scene->lights, RX_ERR_INVALID, "scene has lights but lights is NULL");
-    if (scene->n_tris > 0) {
-        RX_REQUIRE(scene->nodes && scene->header && scene->tris && scene->tri_shade && scene->materials,
-                   RX_ERR_INVALID, "scene has triangles but no BVH/shading arrays");
-        RX_REQUIRE(scene->bvh_depth >= 0 && scene->bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY,
-                   "BVH depth %d exceeds %d", scene->bvh_depth, RX_MAX_BVH_DEPTH);
-    }
-    if (n_samples == 0) return RX_OK;
-    int spp = params->samples_per_pass;
-    size_t need = 0;
-    int st = rx_render_workspace_size(camera->width, camera->height, spp, scene->n_lights, &need);
+    rx_job j;
+    j.scene = *scene;
+    j.camera = *camera;
+    j.params = *params;
+    j.accum = *accum;
+    j.first_sample = first_sample;
+    j.samples_left = n_samples;
+    j.rays = 0;
+    long long P = 0, NP = 0;
+    pass_plan pl;
+    int32_t poff[2], qoff[2];
+    int st = plan_jobs(&j, 1, &P, &NP, &pl, poff, qoff);
     if (st) return st;
-    RX_REQUIRE(d_workspace && workspace_bytes >= need, RX_ERR_WORKSPACE,
-               "render workspace too small: need %zu bytes, have %zu", need, workspace_bytes);
-    rx_arena ar{reinterpret_cast<char*>(d_workspace), workspace_bytes, 0};
-    ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
+    j.path_offset = 0;
+    j.pixel_offset = 0;
+    if (n_samples == 0) return RX_OK;
     ws_view w;
-    carve(ar, (long long)camera->width * camera->height * spp, w);
-
+    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, w);
+    if (st) return st;
     hipStream_t s = rx_stream(stream);
+    k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
+    RX_CHECK_LAUNCH();
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
     tm.s = s;
-    bool deep = scene->n_tris > 0 && use_deep_stack(scene->bvh_depth);
-    int done = 0;
-    while (done < n_samples) {
-        int ns = n_samples - done < spp ? n_samples - done : spp;
-        if (deep)
-            st = run_pass<STACK_DEEP, BS_DEEP>(*scene, *camera, *params, first_sample + done, ns, w, *accum,
-                                               d_counters, tm, s);
-        else
-            st = run_pass<STACK_SHALLOW, BS_SHALLOW>(*scene, *camera, *params, first_sample + done, ns, w, *accum,
-                                                     d_counters, tm, s);
+    int spp = params->samples_per_pass;
+    for (int done = 0; done < n_samples; done += spp) {
+        st = enqueue(w, pl, d_counters, tm, s);
         if (st) return st;
         if (profile) profile->n_passes += 1;
-        done += ns;
         if (tm.on) {
             st = tm.finish(profile);
             if (st) return st;
         }
     }
+    return RX_OK;
+}
+
+extern "C" int rx_batch_workspace_size(const rx_job* jobs, int n_jobs, size_t* bytes) {
+    RX_REQUIRE(bytes, RX_ERR_INVALID, "bytes is NULL");
+    long long P = 0, NP = 0;
+    pass_plan pl;
+    int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, nullptr, nullptr);
+    if (st) return st;
+    *bytes = workspace_bytes_for(n_jobs, P);
+    return RX_OK;
+}
+
+extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace, size_t workspace_bytes,
+                               int use_graph, rx_batch** out) {
+    RX_REQUIRE(out, RX_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    RX_REQUIRE(jobs && n_jobs >= 1 && n_jobs <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
+    int32_t* poff = (int32_t*)malloc(sizeof(int32_t) * (n_jobs + 1));
+    int32_t* qoff = (int32_t*)malloc(sizeof(int32_t) * (n_jobs + 1));
+    rx_job* hj = (rx_job*)malloc(sizeof(rx_job) * n_jobs);
+    RX_REQUIRE(poff && qoff && hj, RX_ERR_INVALID, "out of host memory");
+    long long P = 0, NP = 0;
+    pass_plan pl;
+    int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, poff, qoff);
+    ws_view w;
+    if (!st) st = carve_into(d_workspace, workspace_bytes, n_jobs, P, NP, w);
+    if (!st) {
+        for (int i = 0; i < n_jobs; ++i) {
+            hj[i] = jobs[i];
+            hj[i].path_offset = poff[i];
+            hj[i].pixel_offset = qoff[i];
+        }
+        hipError_t e = hipMemcpy(w.jobs, hj, sizeof(rx_job) * n_jobs, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w.path_off, poff, sizeof(int32_t) * (n_jobs + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w.pix_off, qoff, sizeof(int32_t) * (n_jobs + 1), hipMemcpyHostToDevice);
+        if (e != hipSuccess) st = rx::set_error(RX_ERR_HIP, "job upload failed: %s", hipGetErrorString(e));
+    }
+    free(poff);
+    free(qoff);
+    free(hj);
+    if (st) return st;
+    rx_batch* b = new (std::nothrow) rx_batch;
+    RX_REQUIRE(b, RX_ERR_INVALID, "out of host memory");
+    b->w = w;
+    b->plan = pl;
+    b->n_jobs = n_jobs;
+    b->use_graph = use_graph != 0;
+    b->exec = nullptr;
+    b->captured = false;
+    b->graph_ctr = nullptr;
+    *out = b;
+    return RX_OK;
+}
+
+extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx_profile* profile, void* stream) {
+    RX_REQUIRE(b && d_counters, RX_ERR_INVALID, "NULL argument");
+    RX_REQUIRE(passes >= 0, RX_ERR_INVALID, "passes must be >= 0");
+    hipStream_t s = rx_stream(stream);
+    bool prof = profile != nullptr;
+    if (b->use_graph && !prof) {
+        if (b->captured && b->graph_ctr != d_counters) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->captured = false;
+        }
+        if (!b->captured) {
+            hipStream_t cap;
+            RX_TRY_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            stage_timer off;
+            off.on = false;
+            hipGraph_t g;
+            hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            int st = RX_OK;
+            if (e == hipSuccess) st = enqueue(b->w, b->plan, d_counters, off, cap);
+            hipError_t e2 = hipStreamEndCapture(cap, &g);
+            (void)hipStreamDestroy(cap);
+            if (st) return st;
+            if (e != hipSuccess || e2 != hipSuccess)
+                return rx::set_error(RX_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(e),
+                                     hipGetErrorString(e2));
+            hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e3 != hipSuccess)
+                return rx::set_error(RX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e3));
+            b->captured = true;
+            b->graph_ctr = d_counters;
+        }
+        for (int k = 0; k < passes; ++k) RX_TRY_HIP(hipGraphLaunch(b->exec, s));
+        return RX_OK;
+    }
+    stage_timer tm;
+    tm.on = prof;
+    tm.s = s;
+    for (int k = 0; k < passes; ++k) {
+        int st = enqueue(b->w, b->plan, d_counters, tm, s);
+        if (st) return st;
+        if (profile) profile->n_passes += 1;
+        if (tm.on) {
+            st = tm.finish(profile);
+            if (st) return st;
+        }
+    }
+    return RX_OK;
+}
+
+extern "C" int rx_batch_read_jobs(rx_batch* b, rx_job* jobs_out, void* stream) {
+    RX_REQUIRE(b && jobs_out, RX_ERR_INVALID, "NULL argument");
+    hipStream_t s = rx_stream(stream);
+    RX_TRY_HIP(hipStreamSynchronize(s));
+    RX_TRY_HIP(hipMemcpy(jobs_out, b->w.jobs, sizeof(rx_job) * b->n_jobs, hipMemcpyDeviceToHost));
+    return RX_OK;
+}
+
+extern "C" int rx_batch_destroy(rx_batch* b) {
+    if (!b) return RX_OK;
+    if (b->captured) (void)hipGraphExecDestroy(b->exec);
+    delete b;
     return RX_OK;
 }

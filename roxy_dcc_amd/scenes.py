@@ -139,29 +139,24 @@ def geodesic_sphere(freq):
 def uv_sphere(seg_u, seg_v):
     """UV sphere with seg_u longitudes and seg_v latitude bands
     (2 * seg_u * (seg_v - 1) triangles; poles as triangle fans)."""
-    pos = [[0.0, 1.0, 0.0]]
-    for i in range(1, seg_v):
-        th = math.pi * i / seg_v
-        for j in range(seg_u):
-            ph = 2 * math.pi * j / seg_u
-            pos.append([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)])
-    pos.append([0.0, -1.0, 0.0])
-    pos = np.asarray(pos)
+    th = np.pi * np.arange(1, seg_v) / seg_v
+    ph = 2 * np.pi * np.arange(seg_u) / seg_u
+    ring = np.stack([np.outer(np.sin(th), np.cos(ph)), np.repeat(np.cos(th)[:, None], seg_u, axis=1),
+                     np.outer(np.sin(th), np.sin(ph))], axis=-1).reshape(-1, 3)
+    pos = np.concatenate([[[0.0, 1.0, 0.0]], ring, [[0.0, -1.0, 0.0]]])
     south = len(pos) - 1
-    tris = []
+    j = np.arange(seg_u)
+    jn = (j + 1) % seg_u
 
-    def ring(i, j):
-        return 1 + (i - 1) * seg_u + (j % seg_u)
+    def rid(i, jj):
+        return 1 + (i - 1) * seg_u + jj
 
-    for j in range(seg_u):
-        tris.append([0, ring(1, j + 1), ring(1, j)])
-    for i in range(1, seg_v - 1):
-        for j in range(seg_u):
-            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j + 1), ring(i + 1, j)
-            tris += [[a, b, c], [a, c, d]]
-    for j in range(seg_u):
-        tris.append([south, ring(seg_v - 1, j), ring(seg_v - 1, j + 1)])
-    return pos, np.asarray(tris, dtype=np.int32)
+    north = np.stack([np.zeros(seg_u, np.int64), rid(1, jn), rid(1, j)], axis=1)
+    i = np.arange(1, seg_v - 1)[:, None]
+    a, b, c, d = rid(i, j), rid(i, jn), rid(i + 1, jn), rid(i + 1, j)
+    band = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], axis=2).reshape(-1, 3)
+    southf = np.stack([np.full(seg_u, south), rid(seg_v - 1, j), rid(seg_v - 1, jn)], axis=1)
+    return pos, np.concatenate([north, band, southf]).astype(np.int32)
 
 
 def displaced(pos, rng, amplitude=0.05):

@@ -72,7 +72,7 @@ def test_struct_layouts_match_header(tmp_path):
 
     structs = {"rx_scene": nat.RxScene, "rx_camera": nat.RxCamera, "rx_render_params": nat.RxRenderParams,
                "rx_accum": nat.RxAccum, "rx_counters": nat.RxCounters, "rx_profile": nat.RxProfile,
-               "rx_bvh_ref": nat.RxBvhRef}
+               "rx_bvh_ref": nat.RxBvhRef, "rx_job": nat.RxJob}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "roxy_hip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
