@@ -19,6 +19,7 @@ Prints one JSON line:
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -81,6 +82,98 @@ def cpu_baseline(name, target_seconds=15.0, threads=None):
                       f"{elapsed:.1f} s"}
 
 
+def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_pass=None):
+    """Time `steps` full renders of one scene (profiled), plus one counted render."""
+    spp = st["samples"]
+    per_pass = per_pass or R.default_samples_per_pass(W, H, spp)
+    integ = R.Integrator(sc, W, H, per_pass)
+    frame = R.DeviceFrame(W, H)
+    cam = R.camera_struct(world.active_camera, W, H)
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+
+    def render(flags):
+        prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                              world._background_color, 0, per_pass, flags)
+        done = 0
+        while done < spp:
+            n = min(per_pass, spp - done)
+            integ.render(frame, cam, prm, done, n)
+            done += n
+
+    render(0)
+    integ.reset_counts()
+    render(nat.RX_FLAG_COUNT_TRAFFIC)
+    torch.cuda.synchronize()
+    work = integ.counts()
+    integ.reset_counts()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        render(flags_time)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return integ, frame, work, dt, per_pass
+
+
+def _roofline(work, prof, steps, traffic):
+    alg = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"] + PRIM_B * work["prim_tests"]
+    launches = prof.n_extend / steps
+    bytes_per_launch = alg / max(launches, 1)
+    ms = prof.ms_extend / max(prof.n_extend, 1)
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_extend (closest-hit traversal)",
+            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "launches_per_step": launches}
+
+
+def large_scene_extras(steps=2):
+    """The 2M-triangle 1080p scene: GPU BVH build time, traversal roofline, Mrays/s."""
+    import torch
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.bvh import GPUBVHBuilder
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, W, H, st = scenes.large_scene()
+    t0 = time.perf_counter()
+    ex = extract_scene(world)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    sc = ex.scene
+    # GPU BVH build alone, triangles already resident
+    b = GPUBVHBuilder()
+    tb = ex.flat.triangles()
+    b.build([tb])
+    torch.cuda.synchronize()
+    ws = C.c_size_t()
+    n = tb.triangle_count
+    nat.check(nat.lib().rx_bvh_workspace_size(n, 8, C.byref(ws)))
+    work_buf = torch.empty(ws.value, dtype=torch.uint8, device="cuda")
+    ref = b._ref_struct()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    ev0.record()
+    for _ in range(reps):
+        nat.check(nat.lib().rx_bvh_build_morton(b.d["v0"].data_ptr(), b.d["v1"].data_ptr(), b.d["v2"].data_ptr(), n,
+                                                8, work_buf.data_ptr(), ws.value, C.byref(ref),
+                                                b.d["nodes"].data_ptr(), b.d["header"].data_ptr(),
+                                                b.d["order"].data_ptr(), nat.stream_ptr()))
+    ev1.record()
+    torch.cuda.synchronize()
+    build_ms = ev0.elapsed_time(ev1) / reps
+    integ, frame, work, dt, per_pass = _render_measure(R, nat, torch, world, W, H, st, sc, steps, nat.RX_FLAG_PROFILE)
+    counts = integ.counts()
+    return {"workload": f"large: {W}x{H}, {st['samples']} spp, depth {st['max_depth']}, {sc.n_tris} triangles, "
+                        f"{sc.n_prims} primitives, {sc.n_lights} lights",
+            "bvh_nodes": sc.builder.node_count, "bvh_depth": sc.builder.depth,
+            "gpu_bvh_build_ms": build_ms, "scene_setup_s": setup,
+            "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
+            "samples_per_s": W * H * st["samples"] * steps / dt,
+            "roofline": _roofline(work, integ.profile, steps, _load_traffic("large")),
+            "traversal_work_per_render": work}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +183,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-pass", type=int, default=None)
+    ap.add_argument("--no-large", action="store_true", help="skip the 2M-triangle 1080p scene extras")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -204,6 +298,8 @@ def main():
                      "launches_per_step": ext_launches_step},
         "traversal_work_per_step": work,
     }
+    if not args.no_large and args.config != "large":
+        out["large_scene"] = large_scene_extras()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -290,10 +290,10 @@ int rx_any_hit(const rx_scene* scene, const float* d_rays, const float* d_tmax,
  * Renders samples [first_sample, first_sample + n_samples) and folds them
  * into *accum in sample order with trace.py's running mean.  The
  * workspace must hold rx_render_workspace_size() bytes for the same
- * width, height and params->samples_per_pass.  *profile is written (and
+ * width, height, params->samples_per_pass and scene->bvh_depth.  *profile is written (and
  * the call synchronises) only when RX_FLAG_PROFILE is set. */
 int rx_render_workspace_size(int width, int height, int samples_per_pass,
-                             int n_lights, size_t* bytes);
+                             int bvh_depth, size_t* bytes);
 int rx_render(const rx_scene* scene, const rx_camera* camera,
               const rx_render_params* params, int first_sample, int n_samples,
               void* d_workspace, size_t workspace_bytes,

@@ -289,27 +289,92 @@ struct trav_count {
     uint32_t pops, tris, prims;
 };
 
-// The LDS stack of one lane: entry k lives at base[k * STRIDE] as
-// (link, float bits of t_enter); STRIDE = threads per block keeps the 64
-// lanes of a wave on distinct banks whatever their stack depths.
-template <int STRIDE>
+// A lane's traversal stack of (link, float bits of t_enter) pairs.  The
+// first SL entries live in LDS at lds[k * STRIDE] (STRIDE = threads per
+// block, so the 64 lanes of a wave sit on distinct banks whatever their
+// depths); deeper entries go to the lane's column of a global spill area,
+// spill[(k - SL) * spill_stride].  It is one LIFO either way, so the visit
+// order does not depend on SL.  SL = 0 keeps every entry in LDS.
+template <int STRIDE, int SL>
 struct lane_stack {
-    int2* base;
-    __device__ __forceinline__ void put(int k, int link, float t) { base[k * STRIDE] = make_int2(link, __float_as_int(t)); }
-    __device__ __forceinline__ int2 get(int k) const { return base[k * STRIDE]; }
+    int2* lds;
+    int2* spill;
+    int spill_stride;
+    __device__ __forceinline__ void put(int k, int link, float t) {
+        int2 v = make_int2(link, __float_as_int(t));
+        if (SL == 0 || k < SL)
+            lds[k * STRIDE] = v;
+        else
+            spill[(k - SL) * spill_stride] = v;
+    }
+    __device__ __forceinline__ int2 get(int k) const {
+        if (SL == 0 || k < SL) return lds[k * STRIDE];
+        return spill[(k - SL) * spill_stride];
+    }
 };
+
+// Both child boxes of an interior node, pushed far child first (equal
+// entry t: left pushed first, so the right child pops first) —
+// rendering/taichi/intersect.py:120-150.
+template <class Stack>
+__device__ __forceinline__ void visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
+                                               float limit, Stack& stk, int& sp) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + link);
+    float4 a = p[0], b = p[1], c = p[2];
+    int4 d = reinterpret_cast<const int4*>(p)[3];
+    float lt, rt;
+    bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
+    bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
+    if (lh && rh) {
+        if (lt < rt) {
+            stk.put(sp++, d.y, rt);
+            stk.put(sp++, d.x, lt);
+        } else {
+            stk.put(sp++, d.x, lt);
+            stk.put(sp++, d.y, rt);
+        }
+    } else if (lh) {
+        stk.put(sp++, d.x, lt);
+    } else if (rh) {
+        stk.put(sp++, d.y, rt);
+    }
+}
+
+// Pops the next entry whose t_enter does not exceed `limit` (entries
+// beyond it are discarded, intersect.py:107-108).  Returns false when the
+// stack runs empty.
+template <class Stack>
+__device__ __forceinline__ bool pop_next(const Stack& stk, int& sp, float limit, int& link) {
+    while (sp > 0) {
+        int2 e = stk.get(--sp);
+        if (__int_as_float(e.y) > limit) continue;
+        link = e.x;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void load_tri(const rx_tri* __restrict__ tris, int i, f3& v0, f3& e1, f3& e2) {
+    const float4* tp = reinterpret_cast<const float4*>(tris + i);
+    float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    v0 = mk(p0.x, p0.y, p0.z);
+    e1 = mk(p0.w, p1.x, p1.y);
+    e2 = mk(p1.z, p1.w, p2.x);
+}
 
 // Closest hit over the BVH: rendering/taichi/intersect.py:81-152.  Visit
 // order, pruning and acceptance follow the reference exactly: pop, skip if
 // the stored entry t exceeds the current closest t, test a leaf's
 // triangles in ascending order with 0.001 < t < closest, or test both
-// child boxes against the current closest and push the far child first
-// (equal entry t: left pushed first, so the right child pops first).
-template <int STRIDE, bool COUNT>
+// child boxes against the current closest and push them far child first.
+// The loop is arranged "while-while": a lane runs interior nodes until it
+// holds a leaf, then the wave's leaves are intersected together.  That
+// changes only how lanes interleave, never a lane's own sequence of pops,
+// tests and pushes.
+template <class Stack, bool COUNT>
 __device__ __forceinline__ void bvh_closest(const rx_node* __restrict__ nodes, const rx_tri* __restrict__ tris,
-                                            const rx_bvh_header& hdr, f3 ro, f3 rd, lane_stack<STRIDE> stk,
-                                            float& best_t, int& best_i, float& best_u, float& best_v,
-                                            trav_count& cnt) {
+                                            const rx_bvh_header& hdr, f3 ro, f3 rd, Stack stk, float& best_t,
+                                            int& best_i, float& best_u, float& best_v, trav_count& cnt) {
     f3 inv = inv_dir(rd);
     best_t = 1e9f;
     best_i = -1;
@@ -318,22 +383,25 @@ __device__ __forceinline__ void bvh_closest(const rx_node* __restrict__ nodes, c
     float rt;
     if (!slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2], best_t, &rt))
         return;
+    // the root is pushed and popped at once; its t_enter <= 1e9 = closest
     int sp = 0;
-    stk.put(sp++, hdr.root_link, rt);
-    while (sp > 0) {
-        int2 e = stk.get(--sp);
-        float nt = __int_as_float(e.y);
-        if (nt > best_t) continue;
-        if (COUNT) cnt.pops++;
-        int link = e.x;
-        if (link < 0) {
-            int w = ~link;
+    int cur = hdr.root_link;
+    bool have = true;
+    while (have) {
+        while (have && cur >= 0) {
+            if (COUNT) cnt.pops++;
+            visit_interior(nodes, cur, ro, inv, best_t, stk, sp);
+            have = pop_next(stk, sp, best_t, cur);
+        }
+        if (have) {
+            if (COUNT) cnt.pops++;
+            int w = ~cur;
             int start = w >> 8, end = start + (w & 255);
             for (int i = start; i < end; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(tris + i);
-                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                f3 v0, e1, e2;
+                load_tri(tris, i, v0, e1, e2);
                 float u, v;
-                float t = tri_hit(ro, rd, mk(p0.x, p0.y, p0.z), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), &u, &v);
+                float t = tri_hit(ro, rd, v0, e1, e2, &u, &v);
                 if (COUNT) cnt.tris++;
                 if (0.001f < t && t < best_t) {
                     best_t = t;
@@ -342,79 +410,43 @@ __device__ __forceinline__ void bvh_closest(const rx_node* __restrict__ nodes, c
                     best_v = v;
                 }
             }
-        } else {
-            const float4* p = reinterpret_cast<const float4*>(nodes + link);
-            float4 a = p[0], b = p[1], c = p[2];
-            int4 d = reinterpret_cast<const int4*>(p)[3];
-            float lt, rt2;
-            bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, best_t, &lt);
-            bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, best_t, &rt2);
-            if (lh && rh) {
-                if (lt < rt2) {
-                    stk.put(sp++, d.y, rt2);
-                    stk.put(sp++, d.x, lt);
-                } else {
-                    stk.put(sp++, d.x, lt);
-                    stk.put(sp++, d.y, rt2);
-                }
-            } else if (lh) {
-                stk.put(sp++, d.x, lt);
-            } else if (rh) {
-                stk.put(sp++, d.y, rt2);
-            }
+            have = pop_next(stk, sp, best_t, cur);
         }
     }
 }
 
 // Any hit before max_t: rendering/taichi/intersect.py:155-219 (returns on
 // the first occluder; the boolean does not depend on the visit order).
-template <int STRIDE, bool COUNT>
+template <class Stack, bool COUNT>
 __device__ __forceinline__ bool bvh_occluded(const rx_node* __restrict__ nodes, const rx_tri* __restrict__ tris,
-                                             const rx_bvh_header& hdr, f3 ro, f3 rd, float max_t,
-                                             lane_stack<STRIDE> stk, trav_count& cnt) {
+                                             const rx_bvh_header& hdr, f3 ro, f3 rd, float max_t, Stack stk,
+                                             trav_count& cnt) {
     f3 inv = inv_dir(rd);
     float rt;
     if (!slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2], max_t, &rt))
         return false;
     int sp = 0;
-    stk.put(sp++, hdr.root_link, rt);
-    while (sp > 0) {
-        int2 e = stk.get(--sp);
-        float nt = __int_as_float(e.y);
-        if (nt > max_t) continue;
-        if (COUNT) cnt.pops++;
-        int link = e.x;
-        if (link < 0) {
-            int w = ~link;
+    int cur = hdr.root_link;
+    bool have = rt <= max_t;
+    while (have) {
+        while (have && cur >= 0) {
+            if (COUNT) cnt.pops++;
+            visit_interior(nodes, cur, ro, inv, max_t, stk, sp);
+            have = pop_next(stk, sp, max_t, cur);
+        }
+        if (have) {
+            if (COUNT) cnt.pops++;
+            int w = ~cur;
             int start = w >> 8, end = start + (w & 255);
             for (int i = start; i < end; ++i) {
-                const float4* tp = reinterpret_cast<const float4*>(tris + i);
-                float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+                f3 v0, e1, e2;
+                load_tri(tris, i, v0, e1, e2);
                 float u, v;
-                float t = tri_hit(ro, rd, mk(p0.x, p0.y, p0.z), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), &u, &v);
+                float t = tri_hit(ro, rd, v0, e1, e2, &u, &v);
                 if (COUNT) cnt.tris++;
                 if (0.001f < t && t < max_t) return true;
             }
-        } else {
-            const float4* p = reinterpret_cast<const float4*>(nodes + link);
-            float4 a = p[0], b = p[1], c = p[2];
-            int4 d = reinterpret_cast<const int4*>(p)[3];
-            float lt, rt2;
-            bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, max_t, &lt);
-            bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, max_t, &rt2);
-            if (lh && rh) {
-                if (lt < rt2) {
-                    stk.put(sp++, d.y, rt2);
-                    stk.put(sp++, d.x, lt);
-                } else {
-                    stk.put(sp++, d.x, lt);
-                    stk.put(sp++, d.y, rt2);
-                }
-            } else if (lh) {
-                stk.put(sp++, d.x, lt);
-            } else if (rh) {
-                stk.put(sp++, d.y, rt2);
-            }
+            have = pop_next(stk, sp, max_t, cur);
         }
     }
     return false;
@@ -430,9 +462,9 @@ struct hit_rec {
     float u, v;
 };
 
-template <int STRIDE, bool COUNT>
+template <class Stack, bool COUNT>
 __device__ __forceinline__ hit_rec scene_closest(const rx_scene& sc, const rx_bvh_header& hdr, f3 ro, f3 rd,
-                                                 lane_stack<STRIDE> stk, trav_count& cnt) {
+                                                 Stack stk, trav_count& cnt) {
     hit_rec h{1e9f, -1, RX_KIND_MISS, 0.0f, 0.0f};
     for (int i = 0; i < sc.n_prims; ++i) {
         float t = prim_hit(sc.prims[i], ro, rd);
@@ -446,7 +478,7 @@ __device__ __forceinline__ hit_rec scene_closest(const rx_scene& sc, const rx_bv
     if (sc.n_tris > 0) {
         float t, u, v;
         int idx;
-        bvh_closest<STRIDE, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, stk, t, idx, u, v, cnt);
+        bvh_closest<Stack, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, stk, t, idx, u, v, cnt);
         if (0.001f < t && t < h.t) {
             h.t = t;
             h.id = idx;
@@ -459,15 +491,15 @@ __device__ __forceinline__ hit_rec scene_closest(const rx_scene& sc, const rx_bv
 }
 
 // rendering/taichi/intersect.py:222-246
-template <int STRIDE, bool COUNT>
+template <class Stack, bool COUNT>
 __device__ __forceinline__ bool scene_occluded(const rx_scene& sc, const rx_bvh_header& hdr, f3 ro, f3 rd,
-                                               float max_t, lane_stack<STRIDE> stk, trav_count& cnt) {
+                                               float max_t, Stack stk, trav_count& cnt) {
     for (int i = 0; i < sc.n_prims; ++i) {
         float t = prim_hit(sc.prims[i], ro, rd);
         if (COUNT) cnt.prims++;
         if (0.001f < t && t < max_t) return true;
     }
-    if (sc.n_tris > 0) return bvh_occluded<STRIDE, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, max_t, stk, cnt);
+    if (sc.n_tris > 0) return bvh_occluded<Stack, COUNT>(sc.nodes, sc.tris, hdr, ro, rd, max_t, stk, cnt);
     return false;
 }
 

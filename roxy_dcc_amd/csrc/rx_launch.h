@@ -30,12 +30,35 @@ inline int grid_for(long long n, int bs, int per_cu) {
 
 // Stack depth classes: a lane's stack never holds more than depth + 1
 // entries (one deferred sibling per level plus the two children just
-// pushed), so trees up to depth 23 use 24 entries and deeper ones 64.
+// pushed).  Trees up to depth 23 get 24 entries, deeper ones 64.  The first
+// STACK_LDS entries live in LDS (8 x 8 B per lane: 16 KB per 256-thread
+// block, so LDS no longer caps occupancy), the rest in a global spill
+// column per thread; traced rays rarely go past 8 entries.
+constexpr int STACK_LDS = 8;
 constexpr int STACK_SHALLOW = 24;
 constexpr int STACK_DEEP = 64;
 constexpr int BS_SHALLOW = 256;
 constexpr int BS_DEEP = 64;
+constexpr int TRAV_BLOCKS_PER_CU_SHALLOW = 8;
+constexpr int TRAV_BLOCKS_PER_CU_DEEP = 32;
 
 inline bool use_deep_stack(int depth) { return depth + 1 > STACK_SHALLOW; }
+
+// Grid of a traversal kernel over up to n rays, and the spill area (int2
+// entries) that grid needs.
+inline int trav_grid(long long n, bool deep) {
+    return deep ? grid_for(n, BS_DEEP, TRAV_BLOCKS_PER_CU_DEEP) : grid_for(n, BS_SHALLOW, TRAV_BLOCKS_PER_CU_SHALLOW);
+}
+inline size_t trav_spill_entries(bool deep) {
+    long long threads = deep ? (long long)device_cu_count() * TRAV_BLOCKS_PER_CU_DEEP * BS_DEEP
+                             : (long long)device_cu_count() * TRAV_BLOCKS_PER_CU_SHALLOW * BS_SHALLOW;
+    // grid_for rounds up to a multiple of 8 blocks
+    threads += 8LL * (deep ? BS_DEEP : BS_SHALLOW);
+    return (size_t)((deep ? STACK_DEEP : STACK_SHALLOW) - STACK_LDS) * (size_t)threads;
+}
+
+// Library-owned spill area for the query entry points (grown on demand,
+// one per device, kept until the process ends).
+int query_spill(bool deep, int2** out);
 
 }  // namespace rx

@@ -56,10 +56,13 @@ struct ws_view {
     float *e_px, *e_py, *e_pz, *e_nx, *e_ny, *e_nz, *e_ax, *e_ay, *e_az, *e_tx, *e_ty, *e_tz, *e_scale;
     // queue lengths: [0], [1] ray queues, [2] next-event queue
     int32_t* counts;
+    // traversal stack spill area (entries past the LDS part)
+    int2* spill;
 };
 
-void carve(rx_arena& ar, int n_jobs, long long P, ws_view& w) {
+void carve(rx_arena& ar, int n_jobs, long long P, bool deep, ws_view& w) {
     w.jobs = ar.take<rx_job>(n_jobs);
+    w.spill = ar.take<int2>(trav_spill_entries(deep));
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
@@ -200,10 +203,11 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
 }
 
 // ---------------------------------------------------------------- extend
-template <int STACK, int SBS, bool COUNT>
+template <int SBS, bool COUNT>
 __global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
-    __shared__ int2 stack_mem[STACK * SBS];
-    lane_stack<SBS> stk{stack_mem + threadIdx.x};
+    __shared__ int2 stack_mem[STACK_LDS * SBS];
+    lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
+                                   (int)gridDim.x * SBS};
     const int n = w.counts[qi];
     // the queue this bounce's shade fills and the next-event queue start empty
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* 
             if (sc.n_tris > 0) hdr = *sc.header;
             f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
             f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
-            hit_rec h = scene_closest<SBS, COUNT>(sc, hdr, ro, rd, stk, cnt);
+            hit_rec h = scene_closest<lane_stack<SBS, STACK_LDS>, COUNT>(sc, hdr, ro, rd, stk, cnt);
             w.h_t[i] = h.t;
             w.h_id[i] = h.id;
             w.h_kind[i] = h.kind;
@@ -485,10 +489,11 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // rendering/taichi/trace.py:60-93 (direct_light_sample) for every queued
 // next-event request, then colour += (throughput * direct) * scale
 // (scale = 1 for diffuse, roughness for glossy, trace.py:199-212).
-template <int STACK, int SBS, bool COUNT>
+template <int SBS, bool COUNT>
 __global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
-    __shared__ int2 stack_mem[STACK * SBS];
-    lane_stack<SBS> stk{stack_mem + threadIdx.x};
+    __shared__ int2 stack_mem[STACK_LDS * SBS];
+    lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
+                                   (int)gridDim.x * SBS};
     const int n = w.counts[2];
     trav_count cnt{0, 0, 0};
     unsigned long long shadow = 0;
@@ -537,7 +542,7 @@ __global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counte
                     if (max_t > 0.001f) {
                         f3 so = hp + nn * 1e-4f;
                         cast++;
-                        if (!scene_occluded<SBS, COUNT>(sc, hdr, so, ld, max_t, stk, cnt)) {
+                        if (!scene_occluded<lane_stack<SBS, STACK_LDS>, COUNT>(sc, hdr, so, ld, max_t, stk, cnt)) {
                             float wgt = all ? 1.0f : (float)L;
                             direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
                         }
@@ -709,24 +714,24 @@ struct pass_plan {
     bool count;              // traversal counters
 };
 
-template <int STACK, int SBS>
+template <int SBS>
 int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
     int gp = grid_for(w.P, BS, 16);
-    int gt = grid_for(w.P, SBS, SBS == 256 ? 16 : 32);
+    int gt = trav_grid(w.P, SBS == BS_DEEP);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, 64, 0, s>>>(w.counts)));
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
         if (pl.count)
-            RX_STAGE(tm, 1, (k_extend<STACK, SBS, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            RX_STAGE(tm, 1, (k_extend<SBS, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
         else
-            RX_STAGE(tm, 1, (k_extend<STACK, SBS, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            RX_STAGE(tm, 1, (k_extend<SBS, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
         RX_STAGE(tm, 2, (k_shade<<<gp, BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             if (pl.count)
-                RX_STAGE(tm, 3, (k_connect<STACK, SBS, true><<<gt, SBS, 0, s>>>(w, d, ctr)));
+                RX_STAGE(tm, 3, (k_connect<SBS, true><<<gt, SBS, 0, s>>>(w, d, ctr)));
             else
-                RX_STAGE(tm, 3, (k_connect<STACK, SBS, false><<<gt, SBS, 0, s>>>(w, d, ctr)));
+                RX_STAGE(tm, 3, (k_connect<SBS, false><<<gt, SBS, 0, s>>>(w, d, ctr)));
         }
     }
     RX_STAGE(tm, 4, (k_accumulate<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
@@ -735,8 +740,8 @@ int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_
 }
 
 int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
-    if (pl.deep) return enqueue_pass<STACK_DEEP, BS_DEEP>(w, pl, ctr, tm, s);
-    return enqueue_pass<STACK_SHALLOW, BS_SHALLOW>(w, pl, ctr, tm, s);
+    if (pl.deep) return enqueue_pass<BS_DEEP>(w, pl, ctr, tm, s);
+    return enqueue_pass<BS_SHALLOW>(w, pl, ctr, tm, s);
 }
 
 int check_job(const rx_job& j, int idx) {
@@ -795,20 +800,20 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
     return RX_OK;
 }
 
-size_t workspace_bytes_for(int n_jobs, long long P) {
+size_t workspace_bytes_for(int n_jobs, long long P, bool deep) {
     rx_arena ar{nullptr, 0, 0};
     ws_view w;
-    carve(ar, n_jobs, P, w);
+    carve(ar, n_jobs, P, deep, w);
     return ar.off + 256;
 }
 
-int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, ws_view& w) {
-    size_t need = workspace_bytes_for(n_jobs, P);
+int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, bool deep, ws_view& w) {
+    size_t need = workspace_bytes_for(n_jobs, P, deep);
     RX_REQUIRE(d_workspace && bytes >= need, RX_ERR_WORKSPACE, "render workspace too small: need %zu bytes, have %zu",
                need, bytes);
     rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, 0};
     ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
-    carve(ar, n_jobs, P, w);
+    carve(ar, n_jobs, P, deep, w);
     w.NP = NP;
     return RX_OK;
 }
@@ -825,14 +830,15 @@ struct rx_batch {
     rx_counters* graph_ctr;
 };
 
-extern "C" int rx_render_workspace_size(int width, int height, int samples_per_pass, int n_lights, size_t* bytes) {
-    (void)n_lights;
+extern "C" int rx_render_workspace_size(int width, int height, int samples_per_pass, int bvh_depth, size_t* bytes) {
     RX_REQUIRE(bytes, RX_ERR_INVALID, "bytes is NULL");
+    RX_REQUIRE(bvh_depth >= 0 && bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY, "BVH depth %d exceeds %d",
+               bvh_depth, RX_MAX_BVH_DEPTH);
     RX_REQUIRE(width > 0 && height > 0 && samples_per_pass > 0, RX_ERR_INVALID,
                "width, height and samples_per_pass must be positive");
     long long P = (long long)width * height * samples_per_pass;
     RX_REQUIRE(P <= 0x7fffffffLL, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
-    *bytes = workspace_bytes_for(1, P);
+    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth));
     return RX_OK;
 }
 
@@ -857,7 +863,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     j.pixel_offset = 0;
     if (n_samples == 0) return RX_OK;
     ws_view w;
-    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, w);
+    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, w);
     if (st) return st;
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
@@ -884,7 +890,7 @@ extern "C" int rx_batch_workspace_size(const rx_job* jobs, int n_jobs, size_t* b
     pass_plan pl;
     int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, nullptr, nullptr);
     if (st) return st;
-    *bytes = workspace_bytes_for(n_jobs, P);
+    *bytes = workspace_bytes_for(n_jobs, P, pl.deep);
     return RX_OK;
 }
 
@@ -901,7 +907,7 @@ extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace
     pass_plan pl;
     int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, poff, qoff);
     ws_view w;
-    if (!st) st = carve_into(d_workspace, workspace_bytes, n_jobs, P, NP, w);
+    if (!st) st = carve_into(d_workspace, workspace_bytes, n_jobs, P, NP, pl.deep, w);
     if (!st) {
         for (int i = 0; i < n_jobs; ++i) {
             hj[i] = jobs[i];

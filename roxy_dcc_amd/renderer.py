@@ -65,7 +65,7 @@ class Integrator:
         self.samples_per_pass = int(samples_per_pass)
         ws = C.c_size_t()
         nat.check(nat.lib().rx_render_workspace_size(self.width, self.height, self.samples_per_pass,
-                                                     scene.n_lights, C.byref(ws)), "render workspace")
+                                                     scene.builder.depth, C.byref(ws)), "render workspace")
         self.workspace = torch.empty(ws.value, dtype=torch.uint8, device=device)
         self.counters = torch.zeros(8, dtype=torch.int64, device=device)
         self.profile = nat.RxProfile()

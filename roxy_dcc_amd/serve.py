@@ -1,0 +1,195 @@
+"""Continuous-batching render server.
+
+Requests (scene, resolution, samples) arrive over time.  The server keeps
+up to ``max_scenes`` scene BVHs resident (least recently used evicted,
+never one an active job needs), admits waiting requests in arrival order
+while the job-slot and HBM budgets allow, and every iteration traces one
+wavefront pass of *all* active jobs together (``Batch``).  Jobs that reach
+their sample target are retired and their frames read back; new arrivals
+join at the next iteration, so their work overlaps with jobs already on the
+GPU.  A job's image is the same alone and in any mix (sampling is keyed by
+job seed, pixel, sample and bounce; accumulation is per job in sample
+order).
+
+``serial_loop`` is the one-job-at-a-time baseline: each request is rendered
+to completion, in arrival order, before the next starts.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .batch import Batch, RenderJob
+
+
+@dataclass
+class Request:
+    rid: int
+    scene_key: object
+    width: int
+    height: int
+    samples: int
+    arrival: float            # seconds after the server clock starts
+    max_depth: int = 6
+    seed: int = 0
+    # filled in by the server
+    admitted: float = None
+    finished: float = None
+    frame: object = field(default=None, repr=False)
+    rays: int = 0
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+class SceneCache:
+    """Resident DeviceScenes keyed by scene key, LRU under a byte budget."""
+
+    def __init__(self, loader, max_scenes=32, max_bytes=64 << 30):
+        self.loader = loader              # key -> (DeviceScene, camera)
+        self.max_scenes = max_scenes
+        self.max_bytes = max_bytes
+        self.entries = collections.OrderedDict()
+        self.loads = 0
+        self.evictions = 0
+
+    def bytes(self):
+        return sum(s.hbm_bytes() for s, _ in self.entries.values())
+
+    def get(self, key, pinned=()):
+        if key in self.entries:
+            self.entries.move_to_end(key)
+            return self.entries[key]
+        scene, cam = self.loader(key)
+        self.loads += 1
+        self.entries[key] = (scene, cam)
+        self._evict(pinned=set(pinned) | {key})
+        return scene, cam
+
+    def _evict(self, pinned):
+        while (len(self.entries) > self.max_scenes or self.bytes() > self.max_bytes):
+            victim = next((k for k in self.entries if k not in pinned), None)
+            if victim is None:
+                break
+            del self.entries[victim]
+            self.evictions += 1
+
+
+def _paths(req, spp_pass):
+    return req.width * req.height * spp_pass
+
+
+class RenderServer:
+    def __init__(self, cache: SceneCache, max_jobs=32, paths_per_pass=1 << 22, job_bytes_budget=32 << 30):
+        self.cache = cache
+        self.max_jobs = max_jobs
+        self.paths_per_pass = paths_per_pass
+        self.job_bytes_budget = job_bytes_budget
+        self.waiting = collections.deque()
+        self.active = []       # (Request, RenderJob)
+        self.done = []
+        self.passes = 0
+
+    def _spp_pass(self, req):
+        return int(max(1, min(req.samples, self.paths_per_pass // max(1, req.width * req.height * 4))))
+
+    @staticmethod
+    def _job_bytes(req, spp_pass):
+        # accumulators (13 floats/pixel) + path state and queues (~47 words/path)
+        return req.width * req.height * 13 * 4 + _paths(req, spp_pass) * 188
+
+    def _admit(self, now):
+        used_paths = sum(_paths(r, j.samples_per_pass) for r, j in self.active)
+        used_bytes = sum(self._job_bytes(r, j.samples_per_pass) for r, j in self.active)
+        while self.waiting and self.waiting[0].arrival <= now and len(self.active) < self.max_jobs:
+            req = self.waiting[0]
+            spp = self._spp_pass(req)
+            need = self._job_bytes(req, spp)
+            if self.active and (used_paths + _paths(req, spp) > self.paths_per_pass * 2
+                                or used_bytes + need > self.job_bytes_budget):
+                break
+            self.waiting.popleft()
+            pinned = [r.scene_key for r, _ in self.active]
+            scene, cam = self.cache.get(req.scene_key, pinned=pinned)
+            job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
+                            samples_per_pass=spp, tag=req)
+            req.admitted = now
+            self.active.append((req, job))
+            used_paths += _paths(req, spp)
+            used_bytes += need
+
+    def run(self, requests, clock=time.perf_counter, idle_sleep=0.0005):
+        """Serve ``requests`` (sorted by arrival); returns the finished list."""
+        import torch
+
+        self.waiting.extend(sorted(requests, key=lambda r: r.arrival))
+        t0 = clock()
+        while self.waiting or self.active:
+            now = clock() - t0
+            self._admit(now)
+            if not self.active:
+                time.sleep(idle_sleep)
+                continue
+            jobs = [j for _, j in self.active]
+            batch = Batch(jobs, use_graph=False)
+            batch.run(1)
+            recs = batch.read_jobs()
+            batch.close()
+            self.passes += 1
+            now = clock() - t0
+            still = []
+            for (req, job), rec in zip(self.active, recs):
+                job.first_sample = rec.first_sample
+                req.rays += int(rec.rays)
+                if rec.samples_left <= 0:
+                    torch.cuda.synchronize()
+                    req.frame = job.frame.color.cpu().numpy()
+                    req.finished = clock() - t0
+                    self.done.append(req)
+                else:
+                    still.append((req, job))
+            self.active = still
+        return self.done
+
+
+def serial_loop(cache: SceneCache, requests, paths_per_pass=1 << 22, clock=time.perf_counter):
+    """One request at a time, in arrival order, each rendered to completion."""
+    import torch
+
+    t0 = clock()
+    out = []
+    for req in sorted(requests, key=lambda r: r.arrival):
+        while clock() - t0 < req.arrival:
+            time.sleep(0.0005)
+        req.admitted = clock() - t0
+        scene, cam = cache.get(req.scene_key)
+        spp = int(max(1, min(req.samples, paths_per_pass // max(1, req.width * req.height))))
+        job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
+                        samples_per_pass=spp, tag=req)
+        b = Batch([job], use_graph=False)
+        b.run()
+        rec = b.read_jobs()[0]
+        b.close()
+        torch.cuda.synchronize()
+        req.rays = int(rec.rays)
+        req.frame = job.frame.color.cpu().numpy()
+        req.finished = clock() - t0
+        out.append(req)
+    return out
+
+
+def poisson_arrivals(rate, n, seed=5):
+    """n arrival times of a Poisson process with ``rate`` requests/second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def latency_stats(done, wall):
+    lat = np.asarray([r.latency for r in done])
+    return {"completed": len(done), "wall_s": wall, "renders_per_s": len(done) / wall if wall > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "rays": int(sum(r.rays for r in done))}

@@ -1,0 +1,55 @@
+"""Error against the fp64 oracle at 4096 spp (tests/golden/rmse_*.npz).
+
+The GPU render (independent seed) must be unbiased against the fp64
+reference means on the fixture's pixels, and its RMSE must fall as
+1/sqrt(spp).  The printed RMSE values are the per-configuration figures."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _gpu_render(name, spp, seed):
+    import torch
+
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, W, H, st = helpers.load(name)
+    ex = extract_scene(world)
+    per = R.default_samples_per_pass(W, H, spp)
+    integ = R.Integrator(ex.scene, W, H, per)
+    frame = R.DeviceFrame(W, H)
+    cam = R.camera_struct(world.active_camera, W, H)
+    prm = R.params_struct(st["max_depth"], int(world._use_sky), 0, st["direct_light_max_depth"], 10.0,
+                          world._background_color, seed, per)
+    done = 0
+    while done < spp:
+        integ.render(frame, cam, prm, done, min(per, spp - done))
+        done += per
+    torch.cuda.synchronize()
+    return frame.color.cpu().numpy().reshape(-1, 3)
+
+
+@pytest.mark.parametrize("name", ["cornell", "mesh"])
+def test_rmse_against_fp64_oracle(name):
+    ref = np.load(GOLDEN / f"rmse_{name}.npz")
+    pix, want = ref["pixels"], ref["color"]
+    rm = {}
+    for spp in (16, 64):
+        got = _gpu_render(name, spp, seed=11)[pix].astype(np.float64)
+        err = got - want
+        rm[spp] = float(np.sqrt(np.mean(err ** 2)))
+        bias = float(err.mean())
+        se = float(err.std() / np.sqrt(err.size))
+        print(f"{name}: {spp} spp RMSE {rm[spp]:.5f} (vs fp64 oracle at {int(ref['spp'])} spp), "
+              f"mean error {bias:.2e} +- {se:.1e}", flush=True)
+        assert abs(bias) < 5 * se + 1e-4
+    assert 1.5 < rm[16] / rm[64] < 2.6

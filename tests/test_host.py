@@ -61,9 +61,12 @@ def test_render_workspace_size_grows_with_paths():
     ws1, ws2 = C.c_size_t(), C.c_size_t()
     assert nat.lib().rx_render_workspace_size(64, 64, 1, 0, C.byref(ws1)) == 0
     assert nat.lib().rx_render_workspace_size(64, 64, 4, 0, C.byref(ws2)) == 0
-    assert ws2.value > 3 * ws1.value
-    # 47 four-byte words per path
-    assert ws1.value >= 64 * 64 * 47 * 4
+    # 49 four-byte words per path, plus the fixed traversal spill area
+    assert ws2.value - ws1.value >= 3 * 64 * 64 * 49 * 4
+    assert ws1.value >= 64 * 64 * 49 * 4
+    wsd = C.c_size_t()
+    assert nat.lib().rx_render_workspace_size(64, 64, 1, 40, C.byref(wsd)) == 0
+    assert wsd.value > ws1.value  # deep trees spill more stack entries
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -251,3 +254,39 @@ def test_denoise_oracle_fixed_point_and_dilation():
     sub = img[10:31:2, 10:31:2]
     dense = denoise_oracle.denoise(sub, 1, 0.3, 1.0)
     np.testing.assert_allclose(d2[12:29:2, 12:29:2], dense[1:-1, 1:-1], rtol=1e-6)
+
+
+# ------------------------------------------------------------------ serving (host logic)
+def test_scene_cache_lru_respects_pins_and_budget():
+    from roxy_dcc_amd.serve import SceneCache
+
+    class Fake:
+        def __init__(self, n):
+            self.n = n
+
+        def hbm_bytes(self):
+            return self.n
+
+    loads = []
+
+    def loader(key):
+        loads.append(key)
+        return Fake(100), None
+
+    c = SceneCache(loader, max_scenes=2, max_bytes=250)
+    c.get("a")
+    c.get("b")
+    c.get("a")           # hit, refreshes a
+    c.get("c")           # evicts b (least recent)
+    assert list(c.entries) == ["a", "c"] and c.evictions == 1
+    c.get("d", pinned=["a", "c"])   # over both budgets, but a and c are pinned
+    assert set(c.entries) == {"a", "c", "d"}
+    assert loads == ["a", "b", "c", "d"]
+
+
+def test_poisson_arrivals_rate():
+    from roxy_dcc_amd.serve import poisson_arrivals
+
+    t = poisson_arrivals(50.0, 20000, seed=1)
+    assert np.all(np.diff(t) > 0)
+    assert abs(len(t) / t[-1] - 50.0) < 2.0
