@@ -203,37 +203,140 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
 }
 
 // ---------------------------------------------------------------- extend
+// Persistent closest-hit kernel with per-lane ray refill.  A fixed grid of
+// waves runs until the queue is drained; whenever lanes of a wave have
+// finished their rays, the wave takes as many new queue slots as it has
+// idle lanes with one atomic (counts[3], zeroed before each extend launch)
+// and hands them out in lane order.  Between refills each busy lane runs
+// its traversal to the next leaf and intersects that leaf, so lanes are
+// never held idle for a whole ray by the slowest ray of the wave.  Each
+// ray's own sequence of pops, tests and pushes is the one of
+// bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
+// unchanged.
 template <int SBS, bool COUNT>
 __global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
     const int n = w.counts[qi];
+    int32_t* fetch = &w.counts[3];
     // the queue this bounce's shade fills and the next-event queue start empty
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         w.counts[1 - qi] = 0;
         w.counts[2] = 0;
     }
+    const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
     unsigned long long rays = 0;
-    const int stride = gridDim.x * SBS;
-    for (int base = xcd_block() * SBS; base < n; base += stride) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            int p = w.q_path[qi][i];
-            const rx_scene& sc = w.jobs[job_of_path(w, p)].scene;
-            rx_bvh_header hdr;
-            if (sc.n_tris > 0) hdr = *sc.header;
-            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
-            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
-            hit_rec h = scene_closest<lane_stack<SBS, STACK_LDS>, COUNT>(sc, hdr, ro, rd, stk, cnt);
-            w.h_t[i] = h.t;
-            w.h_id[i] = h.id;
-            w.h_kind[i] = h.kind;
-            w.h_u[i] = h.u;
-            w.h_v[i] = h.v;
-            w.p_rays[p] += 1;
-            rays++;
+    bool exhausted = false;
+    int i = -1;  // queue slot of this lane's ray, -1 when idle
+    int p = 0;
+    const rx_scene* sc = nullptr;
+    const rx_node* nodes = nullptr;
+    const rx_tri* tris = nullptr;
+    f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, inv = ro;
+    float prim_t = 1e9f, best_t = 1e9f, best_u = 0.0f, best_v = 0.0f;
+    int prim_id = -1, best_i = -1, sp = 0, cur = 0;
+    bool have = false;
+    while (true) {
+        bool idle = i < 0;
+        unsigned long long m = __ballot(idle);
+        if (m != 0ull && !exhausted) {
+            int k = __popcll(m);
+            int base = 0;
+            if (lane == 0) base = atomicAdd(fetch, k);
+            base = __shfl(base, 0, 64);
+            if (base + k >= n) exhausted = true;
+            unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+            int mine = base + __popcll(below);
+            if (idle && mine < n) {
+                i = mine;
+                p = w.q_path[qi][i];
+                sc = &w.jobs[job_of_path(w, p)].scene;
+                ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
+                rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+                rays++;
+                // primitive scan first (intersect.py:259-287)
+                prim_t = 1e9f;
+                prim_id = -1;
+                const int np = sc->n_prims;
+                const rx_prim* prims = sc->prims;
+                for (int k2 = 0; k2 < np; ++k2) {
+                    float t = prim_hit(prims[k2], ro, rd);
+                    if (COUNT) cnt.prims++;
+                    if (t > 0.001f && t < prim_t) {
+                        prim_t = t;
+                        prim_id = k2;
+                    }
+                }
+                best_t = 1e9f;
+                best_i = -1;
+                best_u = 0.0f;
+                best_v = 0.0f;
+                have = false;
+                if (sc->n_tris > 0) {
+                    nodes = sc->nodes;
+                    tris = sc->tris;
+                    const rx_bvh_header hdr = *sc->header;
+                    inv = inv_dir(rd);
+                    float rt;
+                    if (slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2],
+                             best_t, &rt)) {
+                        have = true;
+                        cur = hdr.root_link;
+                        sp = 0;
+                    }
+                }
+            }
+        }
+        if (__ballot(i >= 0) == 0ull) break;
+        if (i >= 0) {
+            if (have) {
+                while (have && cur >= 0) {
+                    if (COUNT) cnt.pops++;
+                    visit_interior(nodes, cur, ro, inv, best_t, stk, sp);
+                    have = pop_next(stk, sp, best_t, cur);
+                }
+                if (have) {
+                    if (COUNT) cnt.pops++;
+                    int lw = ~cur;
+                    int start = lw >> 8, end = start + (lw & 255);
+                    for (int t2 = start; t2 < end; ++t2) {
+                        f3 v0, e1, e2;
+                        load_tri(tris, t2, v0, e1, e2);
+                        float u, v;
+                        float t = tri_hit(ro, rd, v0, e1, e2, &u, &v);
+                        if (COUNT) cnt.tris++;
+                        if (0.001f < t && t < best_t) {
+                            best_t = t;
+                            best_i = t2;
+                            best_u = u;
+                            best_v = v;
+                        }
+                    }
+                    have = pop_next(stk, sp, best_t, cur);
+                }
+            }
+            if (!have) {
+                // intersect.py:292: the BVH wins only if strictly closer
+                float t = prim_t;
+                int id = prim_id, kind = prim_id >= 0 ? RX_KIND_PRIM : RX_KIND_MISS;
+                float u = 0.0f, v = 0.0f;
+                if (0.001f < best_t && best_t < prim_t) {
+                    t = best_t;
+                    id = best_i;
+                    kind = RX_KIND_BVH;
+                    u = best_u;
+                    v = best_v;
+                }
+                w.h_t[i] = t;
+                w.h_id[i] = id;
+                w.h_kind[i] = kind;
+                w.h_u[i] = u;
+                w.h_v[i] = v;
+                w.p_rays[p] += 1;
+                i = -1;
+            }
         }
     }
     wave_add(&ctr->rays, rays);
@@ -298,6 +401,12 @@ __device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
 // ---------------------------------------------------------------- shade
 __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
     const int n = w.counts[qi];
+    // the next bounce's extend hands out its queue from counts[3], this
+    // bounce's connect from counts[4]
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.counts[3] = 0;
+        w.counts[4] = 0;
+    }
     const int qo = 1 - qi;
     const int stride = gridDim.x * BS;
     for (int base = xcd_block() * BS; base < n; base += stride) {
@@ -489,73 +598,165 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // rendering/taichi/trace.py:60-93 (direct_light_sample) for every queued
 // next-event request, then colour += (throughput * direct) * scale
 // (scale = 1 for diffuse, roughness for glossy, trace.py:199-212).
+//
+// Persistent like k_extend: lanes that finish a request take new ones
+// (counts[4], zeroed by shade).  A lane's request walks its lights in
+// order (one light in mode "one", all of them in mode "all"); for each it
+// samples the cone, casts the shadow ray if n.l > 0 and max_t > 0.001, and
+// runs the any-hit traversal a leaf at a time between refills.  The
+// per-request order of lights and the sum order of `direct` are the
+// reference's.
 template <int SBS, bool COUNT>
 __global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
     const int n = w.counts[2];
+    int32_t* fetch = &w.counts[4];
+    const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
     unsigned long long shadow = 0;
-    const int stride = gridDim.x * SBS;
-    for (int base = xcd_block() * SBS; base < n; base += stride) {
-        int i = base + threadIdx.x;
-        if (i < n) {
-            int p = w.e_path[i];
-            int jx = job_of_path(w, p);
-            const rx_job& J = w.jobs[jx];
-            const rx_scene& sc = J.scene;
-            const rx_render_params& prm = J.params;
-            rx_bvh_header hdr;
-            if (sc.n_tris > 0) hdr = *sc.header;
-            int npix = J.camera.width * J.camera.height;
-            int local = p - w.path_off[jx];
-            int pix = local % npix, frame = J.first_sample + local / npix;
-            const int L = sc.n_lights;
-            const bool all = prm.direct_light_mode == 1;
-            f3 hp = mk(w.e_px[i], w.e_py[i], w.e_pz[i]);
-            f3 nn = mk(w.e_nx[i], w.e_ny[i], w.e_nz[i]);
-            f3 alb = mk(w.e_ax[i], w.e_ay[i], w.e_az[i]);
-            u4 r0 = rng(prm, pix, frame, depth, 0u);
-            int sel = (int)(u01(r0.x) * (float)L);
-            if (sel > L - 1) sel = L - 1;
-            int l0 = all ? 0 : sel, l1 = all ? L : sel + 1;
-            f3 direct = mk(0.0f, 0.0f, 0.0f);
-            int cast = 0;
-            for (int li = l0; li < l1; ++li) {
+    bool exhausted = false;
+    int i = -1;  // request of this lane, -1 when idle
+    // request state
+    int p = 0, pix = 0, frame = 0, L = 0, li = 0, l_end = 0, cast = 0;
+    bool all = false;
+    const rx_job* J = nullptr;
+    f3 hp = mk(0.0f, 0.0f, 0.0f), nn = hp, alb = hp, direct = hp;
+    u4 r0{0u, 0u, 0u, 0u};
+    // shadow-ray state
+    bool tracing = false, have = false, occ = false;
+    int sp = 0, cur = 0;
+    f3 so = hp, ld = hp, inv = hp;
+    float max_t = 0.0f, ndotl = 0.0f, solid = 0.0f;
+    while (true) {
+        bool idle = i < 0;
+        unsigned long long m = __ballot(idle);
+        if (m != 0ull && !exhausted) {
+            int k = __popcll(m);
+            int base = 0;
+            if (lane == 0) base = atomicAdd(fetch, k);
+            base = __shfl(base, 0, 64);
+            if (base + k >= n) exhausted = true;
+            unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+            int mine = base + __popcll(below);
+            if (idle && mine < n) {
+                i = mine;
+                p = w.e_path[i];
+                int jx = job_of_path(w, p);
+                J = &w.jobs[jx];
+                int npix = J->camera.width * J->camera.height;
+                int local = p - w.path_off[jx];
+                pix = local % npix;
+                frame = J->first_sample + local / npix;
+                L = J->scene.n_lights;
+                all = J->params.direct_light_mode == 1;
+                hp = mk(w.e_px[i], w.e_py[i], w.e_pz[i]);
+                nn = mk(w.e_nx[i], w.e_ny[i], w.e_nz[i]);
+                alb = mk(w.e_ax[i], w.e_ay[i], w.e_az[i]);
+                r0 = rng(J->params, pix, frame, depth, 0u);
+                int sel = (int)(u01(r0.x) * (float)L);
+                if (sel > L - 1) sel = L - 1;
+                li = all ? 0 : sel;
+                l_end = all ? L : sel + 1;
+                direct = mk(0.0f, 0.0f, 0.0f);
+                cast = 0;
+                tracing = false;
+            }
+        }
+        if (__ballot(i >= 0) == 0ull) break;
+        if (i >= 0) {
+            // start the shadow ray of the next light that casts one
+            while (!tracing && li < l_end) {
                 float xi1, xi2;
                 if (all) {
-                    u4 rl = rng(prm, pix, frame, depth, 2u + (uint32_t)li);
+                    u4 rl = rng(J->params, pix, frame, depth, 2u + (uint32_t)li);
                     xi1 = u01(rl.x);
                     xi2 = u01(rl.y);
                 } else {
                     xi1 = u01(r0.y);
                     xi2 = u01(r0.z);
                 }
-                const rx_light lt = sc.lights[li];
-                f3 ld;
-                float t_light, solid;
+                const rx_light lt = J->scene.lights[li];
+                float t_light;
                 sphere_light_sample(hp, ld3(lt.center), lt.radius, xi1, xi2, &ld, &t_light, &solid);
-                float ndotl = dot(nn, ld);
-                if (ndotl > 0.0f) {
-                    float max_t = t_light * 0.999f;
-                    if (max_t > 0.001f) {
-                        f3 so = hp + nn * 1e-4f;
-                        cast++;
-                        if (!scene_occluded<lane_stack<SBS, STACK_LDS>, COUNT>(sc, hdr, so, ld, max_t, stk, cnt)) {
-                            float wgt = all ? 1.0f : (float)L;
-                            direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
+                ndotl = dot(nn, ld);
+                max_t = t_light * 0.999f;
+                if (ndotl > 0.0f && max_t > 0.001f) {
+                    so = hp + nn * 1e-4f;
+                    cast++;
+                    tracing = true;
+                    // primitive scan, then the BVH (intersect.py:222-246)
+                    occ = false;
+                    const int np = J->scene.n_prims;
+                    const rx_prim* prims = J->scene.prims;
+                    for (int k2 = 0; k2 < np && !occ; ++k2) {
+                        float t = prim_hit(prims[k2], so, ld);
+                        if (COUNT) cnt.prims++;
+                        if (0.001f < t && t < max_t) occ = true;
+                    }
+                    have = false;
+                    if (!occ && J->scene.n_tris > 0) {
+                        const rx_bvh_header hdr = *J->scene.header;
+                        inv = inv_dir(ld);
+                        float rt;
+                        if (slab(so, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1],
+                                 hdr.bmax[2], max_t, &rt)) {
+                            have = true;
+                            cur = hdr.root_link;
+                            sp = 0;
                         }
                     }
+                } else {
+                    ++li;
                 }
             }
-            f3 thr = mk(w.e_tx[i], w.e_ty[i], w.e_tz[i]);
-            f3 add = (thr * direct) * w.e_scale[i];
-            w.col_x[p] = w.col_x[p] + add.x;
-            w.col_y[p] = w.col_y[p] + add.y;
-            w.col_z[p] = w.col_z[p] + add.z;
-            w.p_rays[p] += cast;
-            shadow += cast;
+            if (tracing && have && !occ) {
+                const rx_node* nodes = J->scene.nodes;
+                const rx_tri* tris = J->scene.tris;
+                while (have && cur >= 0) {
+                    if (COUNT) cnt.pops++;
+                    visit_interior(nodes, cur, so, inv, max_t, stk, sp);
+                    have = pop_next(stk, sp, max_t, cur);
+                }
+                if (have) {
+                    if (COUNT) cnt.pops++;
+                    int lw = ~cur;
+                    int start = lw >> 8, end = start + (lw & 255);
+                    for (int t2 = start; t2 < end; ++t2) {
+                        f3 v0, e1, e2;
+                        load_tri(tris, t2, v0, e1, e2);
+                        float u, v;
+                        float t = tri_hit(so, ld, v0, e1, e2, &u, &v);
+                        if (COUNT) cnt.tris++;
+                        if (0.001f < t && t < max_t) {
+                            occ = true;
+                            break;
+                        }
+                    }
+                    if (!occ) have = pop_next(stk, sp, max_t, cur);
+                }
+            }
+            if (tracing && (occ || !have)) {
+                // shadow ray finished: unoccluded lights add their term
+                if (!occ) {
+                    const rx_light lt = J->scene.lights[li];
+                    float wgt = all ? 1.0f : (float)L;
+                    direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
+                }
+                tracing = false;
+                ++li;
+            }
+            if (!tracing && li >= l_end) {
+                f3 thr = mk(w.e_tx[i], w.e_ty[i], w.e_tz[i]);
+                f3 add = (thr * direct) * w.e_scale[i];
+                w.col_x[p] = w.col_x[p] + add.x;
+                w.col_y[p] = w.col_y[p] + add.y;
+                w.col_z[p] = w.col_z[p] + add.z;
+                w.p_rays[p] += cast;
+                shadow += cast;
+                i = -1;
+            }
         }
     }
     wave_add(&ctr->rays, shadow);
