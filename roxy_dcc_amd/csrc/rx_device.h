@@ -313,12 +313,18 @@ struct lane_stack {
     }
 };
 
-// Both child boxes of an interior node, pushed far child first (equal
+// Both child boxes of an interior node, far child pushed first (equal
 // entry t: left pushed first, so the right child pops first) —
-// rendering/taichi/intersect.py:120-150.
+// rendering/taichi/intersect.py:120-150.  The reference then pops the
+// child it pushed last at once; that entry's t_enter passed the same
+// limit and nothing has lowered the limit since, so its pop is never
+// skipped.  The near child is therefore returned in `next` instead of
+// being written to the stack and read back: same visit sequence, one
+// stack round trip fewer per node.  Returns false when neither child box
+// is hit (the caller pops).
 template <class Stack>
-__device__ __forceinline__ void visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
-                                               float limit, Stack& stk, int& sp) {
+__device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
+                                               float limit, Stack& stk, int& sp, int& next) {
     const float4* p = reinterpret_cast<const float4*>(nodes + link);
     float4 a = p[0], b = p[1], c = p[2];
     int4 d = reinterpret_cast<const int4*>(p)[3];
@@ -328,16 +334,22 @@ __device__ __forceinline__ void visit_interior(const rx_node* __restrict__ nodes
     if (lh && rh) {
         if (lt < rt) {
             stk.put(sp++, d.y, rt);
-            stk.put(sp++, d.x, lt);
+            next = d.x;
         } else {
             stk.put(sp++, d.x, lt);
-            stk.put(sp++, d.y, rt);
+            next = d.y;
         }
-    } else if (lh) {
-        stk.put(sp++, d.x, lt);
-    } else if (rh) {
-        stk.put(sp++, d.y, rt);
+        return true;
     }
+    if (lh) {
+        next = d.x;
+        return true;
+    }
+    if (rh) {
+        next = d.y;
+        return true;
+    }
+    return false;
 }
 
 // Pops the next entry whose t_enter does not exceed `limit` (entries
@@ -390,8 +402,7 @@ __device__ __forceinline__ void bvh_closest(const rx_node* __restrict__ nodes, c
     while (have) {
         while (have && cur >= 0) {
             if (COUNT) cnt.pops++;
-            visit_interior(nodes, cur, ro, inv, best_t, stk, sp);
-            have = pop_next(stk, sp, best_t, cur);
+            if (!visit_interior(nodes, cur, ro, inv, best_t, stk, sp, cur)) have = pop_next(stk, sp, best_t, cur);
         }
         if (have) {
             if (COUNT) cnt.pops++;
@@ -431,8 +442,7 @@ __device__ __forceinline__ bool bvh_occluded(const rx_node* __restrict__ nodes, 
     while (have) {
         while (have && cur >= 0) {
             if (COUNT) cnt.pops++;
-            visit_interior(nodes, cur, ro, inv, max_t, stk, sp);
-            have = pop_next(stk, sp, max_t, cur);
+            if (!visit_interior(nodes, cur, ro, inv, max_t, stk, sp, cur)) have = pop_next(stk, sp, max_t, cur);
         }
         if (have) {
             if (COUNT) cnt.pops++;

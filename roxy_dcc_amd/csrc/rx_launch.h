@@ -44,6 +44,13 @@ constexpr int TRAV_BLOCKS_PER_CU_DEEP = 32;
 
 inline bool use_deep_stack(int depth) { return depth + 1 > STACK_SHALLOW; }
 
+// Register budgets of the persistent traversal kernels, as waves per SIMD:
+// the closest-hit kernel fits 8 (64 VGPRs) without spilling, the shadow
+// kernel with its light loop 6 (80 VGPRs).  Dependent node gathers want
+// as many waves in flight as the registers allow.
+#define EXTEND_WAVES_PER_EU 8
+#define CONNECT_WAVES_PER_EU 6
+
 // Grid of a traversal kernel over up to n rays, and the spill area (int2
 // entries) that grid needs.
 inline int trav_grid(long long n, bool deep) {

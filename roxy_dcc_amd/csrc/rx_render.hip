@@ -214,7 +214,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
 template <int SBS, bool COUNT>
-__global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(SBS) void k_extend(ws_view w, int qi, rx_counters* 
             if (have) {
                 while (have && cur >= 0) {
                     if (COUNT) cnt.pops++;
-                    visit_interior(nodes, cur, ro, inv, best_t, stk, sp);
-                    have = pop_next(stk, sp, best_t, cur);
+                    if (!visit_interior(nodes, cur, ro, inv, best_t, stk, sp, cur))
+                        have = pop_next(stk, sp, best_t, cur);
                 }
                 if (have) {
                     if (COUNT) cnt.pops++;
@@ -607,7 +607,7 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // per-request order of lights and the sum order of `direct` are the
 // reference's.
 template <int SBS, bool COUNT>
-__global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAVES_PER_EU))) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
@@ -716,8 +716,8 @@ __global__ __launch_bounds__(SBS) void k_connect(ws_view w, int depth, rx_counte
                 const rx_tri* tris = J->scene.tris;
                 while (have && cur >= 0) {
                     if (COUNT) cnt.pops++;
-                    visit_interior(nodes, cur, so, inv, max_t, stk, sp);
-                    have = pop_next(stk, sp, max_t, cur);
+                    if (!visit_interior(nodes, cur, so, inv, max_t, stk, sp, cur))
+                        have = pop_next(stk, sp, max_t, cur);
                 }
                 if (have) {
                     if (COUNT) cnt.pops++;
