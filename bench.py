@@ -82,7 +82,7 @@ def cpu_baseline(name, target_seconds=15.0, threads=None):
                       f"{elapsed:.1f} s"}
 
 
-def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_pass=None):
+def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_pass=None, extra_flags=0):
     """Time `steps` full renders of one scene (profiled), plus one counted render."""
     spp = st["samples"]
     per_pass = per_pass or R.default_samples_per_pass(W, H, spp)
@@ -93,7 +93,7 @@ def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_p
 
     def render(flags):
         prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
-                              world._background_color, 0, per_pass, flags)
+                              world._background_color, 0, per_pass, flags | extra_flags)
         done = 0
         while done < spp:
             n = min(per_pass, spp - done)
@@ -115,7 +115,12 @@ def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_p
 
 
 def _roofline(work, prof, steps, traffic):
-    alg = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"] + PRIM_B * work["prim_tests"]
+    """k_extend's algorithmic bytes per launch over its average launch time.
+    When the primitive scan ran as its own pre-pass (k_prim_scan, scenes with
+    many analytic primitives) its tests are not k_extend's work."""
+    alg = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"]
+    if prof.n_prim_scan == 0:
+        alg += PRIM_B * work["prim_tests"]
     launches = prof.n_extend / steps
     bytes_per_launch = alg / max(launches, 1)
     ms = prof.ms_extend / max(prof.n_extend, 1)
@@ -125,7 +130,12 @@ def _roofline(work, prof, steps, traffic):
             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "launches_per_step": launches}
 
 
-def large_scene_extras(steps=2):
+def _stages(prof, steps):
+    return {k: getattr(prof, "ms_" + k) / steps for k in
+            ("generate", "prim_scan", "extend", "shade", "connect", "accumulate")}
+
+
+def large_scene_extras(steps=2, extra_flags=0):
     """The 2M-triangle 1080p scene: GPU BVH build time, traversal roofline, Mrays/s."""
     import torch
 
@@ -162,7 +172,8 @@ def large_scene_extras(steps=2):
     ev1.record()
     torch.cuda.synchronize()
     build_ms = ev0.elapsed_time(ev1) / reps
-    integ, frame, work, dt, per_pass = _render_measure(R, nat, torch, world, W, H, st, sc, steps, nat.RX_FLAG_PROFILE)
+    integ, frame, work, dt, per_pass = _render_measure(R, nat, torch, world, W, H, st, sc, steps, nat.RX_FLAG_PROFILE,
+                                                       extra_flags=extra_flags)
     counts = integ.counts()
     return {"workload": f"large: {W}x{H}, {st['samples']} spp, depth {st['max_depth']}, {sc.n_tris} triangles, "
                         f"{sc.n_prims} primitives, {sc.n_lights} lights",
@@ -170,6 +181,7 @@ def large_scene_extras(steps=2):
             "gpu_bvh_build_ms": build_ms, "scene_setup_s": setup,
             "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
             "samples_per_s": W * H * st["samples"] * steps / dt,
+            "stage_ms_per_render": _stages(integ.profile, steps),
             "roofline": _roofline(work, integ.profile, steps, _load_traffic("large")),
             "traversal_work_per_render": work}
 
@@ -184,6 +196,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-pass", type=int, default=None)
     ap.add_argument("--no-large", action="store_true", help="skip the 2M-triangle 1080p scene extras")
+    ap.add_argument("--prim-prepass", action="store_true",
+                    help="run the closest-primitive scan as its own pass (RX_FLAG_PRIM_PREPASS)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -224,10 +238,11 @@ def main():
     cam = R.camera_struct(world.active_camera, W, H)
     mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
     denoise_passes = st.get("denoise_passes", 0)
+    extra_flags = nat.RX_FLAG_PRIM_PREPASS if args.prim_prepass else 0
 
     def step(flags):
         prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
-                              world._background_color, 0, per_pass, flags)
+                              world._background_color, 0, per_pass, flags | extra_flags)
         done = 0
         while done < spp:
             n = min(per_pass, spp - done)
@@ -260,13 +275,7 @@ def main():
     rays = counts["rays"]
     ms_step = dt * 1e3 / args.steps
 
-    n_ext = max(prof.n_extend, 1)
-    alg_bytes_step = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"] + PRIM_B * work["prim_tests"]
-    ext_launches_step = prof.n_extend / args.steps
-    bytes_per_launch = alg_bytes_step / max(ext_launches_step, 1)
-    ms_per_launch = prof.ms_extend / n_ext
-    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
-    traffic = _load_traffic(args.config)
+    roofline = _roofline(work, prof, args.steps, _load_traffic(args.config))
 
     out = {
         "metric": "Mrays/s",
@@ -288,18 +297,13 @@ def main():
         "renders_per_s": args.steps / dt,
         "samples_per_s": W * H * spp * args.steps / dt,
         "rays_per_step": rays // args.steps,
-        "stage_ms_per_step": {k: getattr(prof, "ms_" + k) / args.steps for k in
-                              ("generate", "extend", "shade", "connect", "accumulate")},
+        "stage_ms_per_step": _stages(prof, args.steps),
         "scene_setup_s": t_extract,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_extend (closest-hit traversal)",
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms_per_launch,
-                     "launches_per_step": ext_launches_step},
+        "roofline": roofline,
         "traversal_work_per_step": work,
     }
     if not args.no_large and args.config != "large":
-        out["large_scene"] = large_scene_extras()
+        out["large_scene"] = large_scene_extras(extra_flags=extra_flags)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
     print(json.dumps(out), flush=True)

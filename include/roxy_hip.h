@@ -178,6 +178,8 @@ typedef struct rx_camera {
 /* Flags for rx_render_params.flags */
 #define RX_FLAG_PROFILE 1        /* time each stage with HIP events (synchronises) */
 #define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
+#define RX_FLAG_PRIM_PREPASS 4   /* closest-primitive scan as its own pass over each ray
+                                    queue (automatic above 256 analytic primitives) */
 
 /* Integrator settings (render_kernel arguments, trace.py:239-254). */
 typedef struct rx_render_params {
@@ -225,6 +227,8 @@ typedef struct rx_profile {
     int32_t n_extend;
     int32_t n_connect;
     int32_t n_passes;
+    float ms_prim_scan;     /* primitive pre-pass (scenes with many primitives) */
+    int32_t n_prim_scan;
 } rx_profile;
 
 /* ---- library ---------------------------------------------------------- */

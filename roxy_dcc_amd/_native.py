@@ -29,6 +29,7 @@ RX_MAX_TRIANGLES = 1 << 23
 RX_MAX_BVH_DEPTH = 63
 RX_FLAG_PROFILE = 1
 RX_FLAG_COUNT_TRAFFIC = 2
+RX_FLAG_PRIM_PREPASS = 4
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -81,6 +82,7 @@ class RxProfile(C.Structure):
         ("ms_generate", C.c_float), ("ms_extend", C.c_float), ("ms_shade", C.c_float),
         ("ms_connect", C.c_float), ("ms_accumulate", C.c_float),
         ("n_extend", i32), ("n_connect", i32), ("n_passes", i32),
+        ("ms_prim_scan", C.c_float), ("n_prim_scan", i32),
     ]
 
 

@@ -113,6 +113,21 @@ __device__ __forceinline__ int pass_samples(const rx_job& j) {
     return j.samples_left < spp ? j.samples_left : spp;
 }
 
+// Pixel of slot k (0 <= k < W*H) of a sample's path slots, and back.  When
+// both image sides are multiples of 8, slots run through 8x8 pixel tiles
+// in row-major tile order, so the 64 rays of a wave leave from one
+// compact block of pixels and share node fetches; otherwise slots are
+// scanline order.  Either way a bijection on [0, W*H): sampling is keyed
+// by the pixel, so images do not depend on the slot order.
+__device__ __forceinline__ bool tiled(const rx_camera& c) { return ((c.width | c.height) & 7) == 0; }
+
+__device__ __forceinline__ int slot_pixel(const rx_camera& c, int k) {
+    if (!tiled(c)) return k;
+    int tile = k >> 6, in = k & 63, tiles_x = c.width >> 3;
+    int tx = tile % tiles_x, ty = tile / tiles_x;
+    return (ty * 8 + (in >> 3)) * c.width + tx * 8 + (in & 7);
+}
+
 __device__ __forceinline__ u4 rng(const rx_render_params& prm, int pix, int frame, int depth, uint32_t draw) {
     return philox4x32_10(u4{(uint32_t)pix, (uint32_t)frame, (uint32_t)depth, draw}, prm.seed_lo, prm.seed_hi);
 }
@@ -166,7 +181,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
             const rx_job& J = w.jobs[jx];
             int npix = J.camera.width * J.camera.height;
             long long local = p - w.path_off[jx];
-            int s = (int)(local / npix), pix = (int)(local % npix);
+            int s = (int)(local / npix), pix = slot_pixel(J.camera, (int)(local % npix));
             if (s < pass_samples(J)) {
                 active = true;
                 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
@@ -202,6 +217,53 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     }
 }
 
+// Closest analytic primitive along a ray: the linear scan of
+// rendering/taichi/intersect.py:259-287 (ascending index, strict
+// 0.001 < t < closest, so the lowest index wins a tie).
+template <bool COUNT>
+__device__ __forceinline__ void closest_prim(const rx_scene& sc, f3 ro, f3 rd, float& prim_t, int& prim_id,
+                                             trav_count& cnt) {
+    prim_t = 1e9f;
+    prim_id = -1;
+    const int np = sc.n_prims;
+    const rx_prim* prims = sc.prims;
+    for (int k = 0; k < np; ++k) {
+        float t = prim_hit(prims[k], ro, rd);
+        if (COUNT) cnt.prims++;
+        if (t > 0.001f && t < prim_t) {
+            prim_t = t;
+            prim_id = k;
+        }
+    }
+}
+
+// Closest-primitive pre-pass over ray queue `qi`: writes each ray's
+// primitive hit into h_t / h_id, where k_extend<.., PRE = true> picks it
+// up.  The BVH walk never reads the primitive result (it starts from
+// closest = 1e9 and is compared with it only at the end, intersect.py:292),
+// so computing it first and separately leaves every hit unchanged.
+template <bool COUNT>
+__global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi, rx_counters* __restrict__ ctr) {
+    const int n = w.counts[qi];
+    trav_count cnt{0, 0, 0};
+    const int stride = gridDim.x * BS;
+    for (int base = xcd_block() * BS; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        if (i < n) {
+            int p = w.q_path[qi][i];
+            const rx_scene& sc = w.jobs[job_of_path(w, p)].scene;
+            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
+            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+            float t;
+            int id;
+            closest_prim<COUNT>(sc, ro, rd, t, id, cnt);
+            w.h_t[i] = t;
+            w.h_id[i] = id;
+        }
+    }
+    if (COUNT) wave_add(&ctr->prim_tests, cnt.prims);
+}
+
 // ---------------------------------------------------------------- extend
 // Persistent closest-hit kernel with per-lane ray refill.  A fixed grid of
 // waves runs until the queue is drained; whenever lanes of a wave have
@@ -213,7 +275,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
 // ray's own sequence of pops, tests and pushes is the one of
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
-template <int SBS, bool COUNT>
+template <int SBS, bool COUNT, bool PRE>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
@@ -256,18 +318,13 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
                 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
                 rays++;
-                // primitive scan first (intersect.py:259-287)
-                prim_t = 1e9f;
-                prim_id = -1;
-                const int np = sc->n_prims;
-                const rx_prim* prims = sc->prims;
-                for (int k2 = 0; k2 < np; ++k2) {
-                    float t = prim_hit(prims[k2], ro, rd);
-                    if (COUNT) cnt.prims++;
-                    if (t > 0.001f && t < prim_t) {
-                        prim_t = t;
-                        prim_id = k2;
-                    }
+                // primitive scan first (intersect.py:259-287), or its
+                // result from k_prim_scan
+                if (PRE) {
+                    prim_t = w.h_t[i];
+                    prim_id = w.h_id[i];
+                } else {
+                    closest_prim<COUNT>(*sc, ro, rd, prim_t, prim_id, cnt);
                 }
                 best_t = 1e9f;
                 best_i = -1;
@@ -424,7 +481,7 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
             const rx_render_params& prm = J.params;
             int npix = J.camera.width * J.camera.height;
             int local = p - w.path_off[jx];
-            int pix = local % npix, frame = J.first_sample + local / npix;
+            int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
             bool last = depth + 1 >= prm.max_depth;
             bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
             f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
@@ -647,7 +704,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                 J = &w.jobs[jx];
                 int npix = J->camera.width * J->camera.height;
                 int local = p - w.path_off[jx];
-                pix = local % npix;
+                pix = slot_pixel(J->camera, local % npix);
                 frame = J->first_sample + local / npix;
                 L = J->scene.n_lights;
                 all = J->params.direct_light_mode == 1;
@@ -784,13 +841,14 @@ __global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
             const rx_render_params& prm = J.params;
             const rx_accum& acc = J.accum;
             int npix = J.camera.width * J.camera.height;
-            int pix = (int)(q - w.pix_off[jx]);
+            int k = (int)(q - w.pix_off[jx]);  // slot within a sample
+            int pix = slot_pixel(J.camera, k);
             int ns = pass_samples(J);
             if (ns > 0) {
                 f3 c = ld3(acc.color + 3 * pix), nn = ld3(acc.normal + 3 * pix), aa = ld3(acc.albedo + 3 * pix);
                 float dd = acc.depth[pix];
                 for (int s = 0; s < ns; ++s) {
-                    long long p = (long long)w.path_off[jx] + (long long)s * npix + pix;
+                    long long p = (long long)w.path_off[jx] + (long long)s * npix + k;
                     int frame = J.first_sample + s;
                     f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
                     if (prm.sample_clamp > 0.0f) {
@@ -870,8 +928,8 @@ struct stage_timer {
     int finish(rx_profile* prof) {
         if (!on) return RX_OK;
         RX_TRY_HIP(hipStreamSynchronize(s));
-        float sum[5] = {0, 0, 0, 0, 0};
-        int cnt[5] = {0, 0, 0, 0, 0};
+        float sum[6] = {0, 0, 0, 0, 0, 0};
+        int cnt[6] = {0, 0, 0, 0, 0, 0};
         for (int k = 0; k < n; k += 2) {
             float ms = 0.0f;
             RX_TRY_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
@@ -889,6 +947,8 @@ struct stage_timer {
             prof->ms_accumulate += sum[4];
             prof->n_extend += cnt[1];
             prof->n_connect += cnt[3];
+            prof->ms_prim_scan += sum[5];
+            prof->n_prim_scan += cnt[5];
         }
         return RX_OK;
     }
@@ -913,7 +973,17 @@ struct pass_plan {
     int nee_depth;           // connect launched for depth < nee_depth
     bool deep;               // stack class
     bool count;              // traversal counters
+    bool prim_prepass;       // closest-primitive scan as its own pass
 };
+
+// Scenes with more analytic primitives than this get their closest-
+// primitive scan as a pre-pass over the ray queue (every lane of a wave
+// scans together) instead of inside the persistent traversal kernel,
+// where only the lanes taking a new ray run the loop.  Measured on a
+// 67-primitive scene the inline scan is still faster (the traversal
+// kernel waits on node fetches, not on arithmetic), hence the high bar;
+// RX_FLAG_PRIM_PREPASS asks for the pre-pass at any count.
+constexpr int PRIM_INLINE_MAX = 256;
 
 template <int SBS>
 int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
@@ -923,10 +993,19 @@ int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
-        if (pl.count)
-            RX_STAGE(tm, 1, (k_extend<SBS, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
-        else
-            RX_STAGE(tm, 1, (k_extend<SBS, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+        if (pl.prim_prepass) {
+            if (pl.count) {
+                RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi, ctr)));
+                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            } else {
+                RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi, ctr)));
+                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            }
+        } else if (pl.count) {
+            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+        } else {
+            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+        }
         RX_STAGE(tm, 2, (k_shade<<<gp, BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             if (pl.count)
@@ -976,7 +1055,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false};
+    pass_plan plan{0, 0, false, false, false};
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -991,6 +1070,8 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
             plan.nee_depth = j.params.direct_light_max_depth;
         if (j.scene.n_tris > 0 && use_deep_stack(j.scene.bvh_depth)) plan.deep = true;
         if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
+        if (j.scene.n_prims > 0 && (j.scene.n_prims > PRIM_INLINE_MAX || (j.params.flags & RX_FLAG_PRIM_PREPASS)))
+            plan.prim_prepass = true;
     }
     RX_REQUIRE(p <= 0x7fffffffLL, RX_ERR_CAPACITY, "batch of %lld paths exceeds 2^31-1", p);
     if (path_off) path_off[n] = (int32_t)p;

@@ -79,7 +79,7 @@ def random_rays(flat, n, seed=7):
 
 
 def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0, spp_per_pass=None,
-                direct_light_mode=None, threads=None, world_tuple=None):
+                direct_light_mode=None, threads=None, world_tuple=None, flags=0):
     """Render the same scene with the GPU library and the fp32 oracle."""
     import torch
 
@@ -98,7 +98,8 @@ def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0
     integ = R.Integrator(ex.scene, W, H, spp)
     frame = R.DeviceFrame(W, H)
     camera = R.camera_struct(cam, W, H)
-    params = R.params_struct(md, int(world._use_sky), mode_id, dlmd, 10.0, world._background_color, seed, spp)
+    params = R.params_struct(md, int(world._use_sky), mode_id, dlmd, 10.0, world._background_color, seed, spp,
+                             flags=flags)
     done = 0
     while done < samples:
         n = min(spp, samples - done)

@@ -257,6 +257,27 @@ def test_render_all_lights_mode_matches_oracle():
     assert same.mean() >= 0.99
 
 
+def test_render_prim_prepass_matches_oracle():
+    """Closest-primitive scan as its own pass over each ray queue (forced by
+    RX_FLAG_PRIM_PREPASS; 16 analytic spheres, 8x8 tiled slot order)."""
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import scenes
+
+    world_tuple = scenes.large_scene(seed=4, n_meshes=20, seg_u=20, seg_v=11, n_spheres=16)
+    res = helpers.render_pair("large", width=64, height=40, samples=2, world_tuple=world_tuple,
+                              flags=nat.RX_FLAG_PRIM_PREPASS | nat.RX_FLAG_COUNT_TRAFFIC)
+    same = np.all(res["gpu_color"] == res["cpu_color"], axis=1)
+    g, c = res["gpu_counts"], res["cpu_counts"]
+    print(f"large/prepass: exact {same.mean():.4f} rays {g['rays']} {c['rays']} prim tests {g['prim_tests']} "
+          f"{c['prim_tests']}", flush=True)
+    assert same.mean() >= 0.99
+    assert g["rays"] == c["rays"]
+    if same.all():
+        # same paths, so the same traversal work, pop for pop
+        for k in ("node_pops", "tri_tests", "prim_tests", "shadow_rays", "shadow_prim_tests"):
+            assert g[k] == c[k], k
+
+
 @pytest.mark.parametrize("radius,dilation,guides", [(1, 1, "none"), (1, 1, "all"), (2, 1, "normal"),
                                                     (1, 2, "all"), (1, 4, "albedo"), (1, 16, "depth"),
                                                     (3, 3, "all")])
