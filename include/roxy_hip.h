@@ -145,7 +145,9 @@ typedef struct rx_light {
     float emission;
 } rx_light;
 
-/* Scene view: device pointers plus host-side sizes. */
+/* Scene view: device pointers plus host-side sizes.  Every array must be
+ * 16-byte aligned (records are read as 16-byte words); the entry points
+ * return RX_ERR_INVALID otherwise. */
 typedef struct rx_scene {
     const rx_node* nodes;
     const rx_bvh_header* header;

@@ -43,6 +43,15 @@ static inline hipStream_t rx_stream(void* s) { return reinterpret_cast<hipStream
 
 static inline size_t rx_align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Every scene array is 16-byte aligned (records are read as 16-byte words).
+static inline bool rx_scene_aligned(const rx_scene& sc) {
+    const void* ptrs[] = {sc.nodes, sc.header, sc.tris, sc.tri_shade, sc.materials, sc.prims, sc.prim_materials,
+                          sc.lights};
+    for (const void* p : ptrs)
+        if (reinterpret_cast<uintptr_t>(p) & 15) return false;
+    return true;
+}
+
 // Bump allocator over a caller-provided workspace (256-byte aligned slices).
 struct rx_arena {
     char* base;

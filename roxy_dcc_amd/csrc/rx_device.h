@@ -16,6 +16,31 @@
 
 #pragma clang fp contract(off)
 
+// Address-space qualifiers: pointers read out of scene records are generic
+// to the compiler, and generic (flat) accesses wait on both the LDS and
+// the vector-memory counters.  Casting the hot traversal data to these
+// makes the loads global_load_* and the LDS stack ds_*.
+#define RX_GLOBAL __attribute__((address_space(1)))
+#define RX_LDS __attribute__((address_space(3)))
+typedef float gv4f __attribute__((ext_vector_type(4)));
+typedef int gv4i __attribute__((ext_vector_type(4)));
+
+// Loads a scene record (a multiple of 16 bytes, 16-byte aligned: the entry
+// points check the array alignment) as 16-byte global loads into a local
+// copy, so later field reads do not go back through a generic pointer.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+    static_assert(sizeof(T) % 16 == 0, "scene records are whole 16-byte words");
+    constexpr int N = sizeof(T) / 16;
+    gv4f tmp[N];
+    const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) tmp[i] = q[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
+
 namespace rx {
 
 // ---------------------------------------------------------------- vectors
@@ -295,21 +320,33 @@ struct trav_count {
 // depths); deeper entries go to the lane's column of a global spill area,
 // spill[(k - SL) * spill_stride].  It is one LIFO either way, so the visit
 // order does not depend on SL.  SL = 0 keeps every entry in LDS.
+//
+// Entries are one 64-bit word (link low, t bits high) and the two parts are
+// typed by address space, so the LDS part compiles to ds_read/ds_write_b64
+// and the spill to global loads/stores (a generic pointer selected between
+// them would become flat accesses, which wait on both the LDS and the
+// vector-memory counters).
 template <int STRIDE, int SL>
 struct lane_stack {
-    int2* lds;
-    int2* spill;
+    RX_LDS uint64_t* lds;
+    RX_GLOBAL uint64_t* spill;
     int spill_stride;
+    __device__ __forceinline__ lane_stack(int2* lds_slot, int2* spill_slot, int stride)
+        : lds((RX_LDS uint64_t*)lds_slot), spill((RX_GLOBAL uint64_t*)spill_slot), spill_stride(stride) {}
     __device__ __forceinline__ void put(int k, int link, float t) {
-        int2 v = make_int2(link, __float_as_int(t));
+        uint64_t v = (uint64_t)(uint32_t)link | ((uint64_t)__float_as_uint(t) << 32);
         if (SL == 0 || k < SL)
             lds[k * STRIDE] = v;
         else
             spill[(k - SL) * spill_stride] = v;
     }
     __device__ __forceinline__ int2 get(int k) const {
-        if (SL == 0 || k < SL) return lds[k * STRIDE];
-        return spill[(k - SL) * spill_stride];
+        uint64_t v;
+        if (SL == 0 || k < SL)
+            v = lds[k * STRIDE];
+        else
+            v = spill[(k - SL) * spill_stride];
+        return make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
     }
 };
 
@@ -325,9 +362,9 @@ struct lane_stack {
 template <class Stack>
 __device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
                                                float limit, Stack& stk, int& sp, int& next) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + link);
-    float4 a = p[0], b = p[1], c = p[2];
-    int4 d = reinterpret_cast<const int4*>(p)[3];
+    const RX_GLOBAL gv4f* p = (const RX_GLOBAL gv4f*)(nodes + link);
+    gv4f a = p[0], b = p[1], c = p[2];
+    gv4i d = ((const RX_GLOBAL gv4i*)p)[3];
     float lt, rt;
     bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
     bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
@@ -367,8 +404,8 @@ __device__ __forceinline__ bool pop_next(const Stack& stk, int& sp, float limit,
 }
 
 __device__ __forceinline__ void load_tri(const rx_tri* __restrict__ tris, int i, f3& v0, f3& e1, f3& e2) {
-    const float4* tp = reinterpret_cast<const float4*>(tris + i);
-    float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    const RX_GLOBAL gv4f* tp = (const RX_GLOBAL gv4f*)(tris + i);
+    gv4f p0 = tp[0], p1 = tp[1], p2 = tp[2];
     v0 = mk(p0.x, p0.y, p0.z);
     e1 = mk(p0.w, p1.x, p1.y);
     e2 = mk(p1.z, p1.w, p2.x);
@@ -477,7 +514,7 @@ __device__ __forceinline__ hit_rec scene_closest(const rx_scene& sc, const rx_bv
                                                  Stack stk, trav_count& cnt) {
     hit_rec h{1e9f, -1, RX_KIND_MISS, 0.0f, 0.0f};
     for (int i = 0; i < sc.n_prims; ++i) {
-        float t = prim_hit(sc.prims[i], ro, rd);
+        float t = prim_hit(gload(sc.prims + i), ro, rd);
         if (COUNT) cnt.prims++;
         if (t > 0.001f && t < h.t) {
             h.t = t;
@@ -505,7 +542,7 @@ template <class Stack, bool COUNT>
 __device__ __forceinline__ bool scene_occluded(const rx_scene& sc, const rx_bvh_header& hdr, f3 ro, f3 rd,
                                                float max_t, Stack stk, trav_count& cnt) {
     for (int i = 0; i < sc.n_prims; ++i) {
-        float t = prim_hit(sc.prims[i], ro, rd);
+        float t = prim_hit(gload(sc.prims + i), ro, rd);
         if (COUNT) cnt.prims++;
         if (0.001f < t && t < max_t) return true;
     }

@@ -228,7 +228,7 @@ __device__ __forceinline__ void closest_prim(const rx_scene& sc, f3 ro, f3 rd, f
     const int np = sc.n_prims;
     const rx_prim* prims = sc.prims;
     for (int k = 0; k < np; ++k) {
-        float t = prim_hit(prims[k], ro, rd);
+        float t = prim_hit(gload(prims + k), ro, rd);
         if (COUNT) cnt.prims++;
         if (t > 0.001f && t < prim_t) {
             prim_t = t;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 if (sc->n_tris > 0) {
                     nodes = sc->nodes;
                     tris = sc->tris;
-                    const rx_bvh_header hdr = *sc->header;
+                    const rx_bvh_header hdr = gload(sc->header);
                     inv = inv_dir(rd);
                     float rt;
                     if (slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2],
@@ -504,15 +504,15 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
                 rx_material m;
                 f3 nn;
                 if (kind == RX_KIND_BVH) {
-                    const rx_tri_shade& sh = sc.tri_shade[id];
+                    const rx_tri_shade sh = gload(sc.tri_shade + id);
                     float u = w.h_u[i], v = w.h_v[i];
                     float ww = (1.0f - u) - v;
                     nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
-                    m = sc.materials[sh.mat];
+                    m = gload(sc.materials + sh.mat);
                 } else {
-                    const rx_prim pr = sc.prims[id];
+                    const rx_prim pr = gload(sc.prims + id);
                     nn = prim_normal(pr, ro, rd, t);
-                    m = sc.prim_materials[id];
+                    m = gload(sc.prim_materials + id);
                 }
                 // trace.py:161-165
                 bool front = dot(nn, rd) < 0.0f;
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                     xi1 = u01(r0.y);
                     xi2 = u01(r0.z);
                 }
-                const rx_light lt = J->scene.lights[li];
+                const rx_light lt = gload(J->scene.lights + li);
                 float t_light;
                 sphere_light_sample(hp, ld3(lt.center), lt.radius, xi1, xi2, &ld, &t_light, &solid);
                 ndotl = dot(nn, ld);
@@ -748,13 +748,13 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                     const int np = J->scene.n_prims;
                     const rx_prim* prims = J->scene.prims;
                     for (int k2 = 0; k2 < np && !occ; ++k2) {
-                        float t = prim_hit(prims[k2], so, ld);
+                        float t = prim_hit(gload(prims + k2), so, ld);
                         if (COUNT) cnt.prims++;
                         if (0.001f < t && t < max_t) occ = true;
                     }
                     have = false;
                     if (!occ && J->scene.n_tris > 0) {
-                        const rx_bvh_header hdr = *J->scene.header;
+                        const rx_bvh_header hdr = gload(J->scene.header);
                         inv = inv_dir(ld);
                         float rt;
                         if (slab(so, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1],
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
             if (tracing && (occ || !have)) {
                 // shadow ray finished: unoccluded lights add their term
                 if (!occ) {
-                    const rx_light lt = J->scene.lights[li];
+                    const rx_light lt = gload(J->scene.lights + li);
                     float wgt = all ? 1.0f : (float)L;
                     direct = direct + ((((alb * ld3(lt.albedo)) * lt.emission) * ndotl) * solid) * wgt;
                 }
@@ -1041,6 +1041,7 @@ int check_job(const rx_job& j, int idx) {
     RX_REQUIRE(sc.n_prims == 0 || (sc.prims && sc.prim_materials), RX_ERR_INVALID,
                "job %d: scene has primitives but no primitive arrays", idx);
     RX_REQUIRE(sc.n_lights == 0 || sc.lights, RX_ERR_INVALID, "job %d: scene has lights but lights is NULL", idx);
+    RX_REQUIRE(rx_scene_aligned(sc), RX_ERR_INVALID, "job %d: scene arrays must be 16-byte aligned", idx);
     if (sc.n_tris > 0) {
         RX_REQUIRE(sc.nodes && sc.header && sc.tris && sc.tri_shade && sc.materials, RX_ERR_INVALID,
                    "job %d: scene has triangles but no BVH/shading arrays", idx);

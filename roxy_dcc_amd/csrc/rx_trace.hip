@@ -18,7 +18,7 @@ __global__ __launch_bounds__(BS) void k_closest(rx_scene sc, const float* __rest
     lane_stack<BS, STACK_LDS> stk{stack_mem + threadIdx.x, spill + blockIdx.x * BS + threadIdx.x,
                                   (int)gridDim.x * BS};
     rx_bvh_header hdr;
-    if (sc.n_tris > 0) hdr = *sc.header;
+    if (sc.n_tris > 0) hdr = gload(sc.header);
     trav_count cnt{0, 0, 0};
     const int stride = gridDim.x * BS;
     for (int base = xcd_block() * BS; base < n; base += stride) {
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(BS) void k_any(rx_scene sc, const float* __restrict
     lane_stack<BS, STACK_LDS> stk{stack_mem + threadIdx.x, spill + blockIdx.x * BS + threadIdx.x,
                                   (int)gridDim.x * BS};
     rx_bvh_header hdr;
-    if (sc.n_tris > 0) hdr = *sc.header;
+    if (sc.n_tris > 0) hdr = gload(sc.header);
     trav_count cnt{0, 0, 0};
     const int stride = gridDim.x * BS;
     for (int base = xcd_block() * BS; base < n; base += stride) {
@@ -71,6 +71,7 @@ int check_scene(const rx_scene* sc) {
     RX_REQUIRE(sc, RX_ERR_INVALID, "scene is NULL");
     RX_REQUIRE(sc->n_prims >= 0 && sc->n_tris >= 0 && sc->n_lights >= 0, RX_ERR_INVALID, "negative scene size");
     RX_REQUIRE(sc->n_prims == 0 || sc->prims, RX_ERR_INVALID, "scene has primitives but prims is NULL");
+    RX_REQUIRE(rx_scene_aligned(*sc), RX_ERR_INVALID, "scene arrays must be 16-byte aligned");
     if (sc->n_tris > 0) {
         RX_REQUIRE(sc->nodes && sc->header && sc->tris, RX_ERR_INVALID, "scene has triangles but no BVH");
         RX_REQUIRE(sc->bvh_depth >= 0 && sc->bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY,
