@@ -146,6 +146,40 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     return base + __popcll(below);
 }
 
+// Rays a wave of a persistent kernel keeps in flight at once: all 64 lanes
+// when the queue has work for every resident wave, otherwise an even share
+// (n / waves, at least 1).  When few rays are left (the later bounces),
+// handing 64 of them to each of the first waves would leave most waves
+// idle and every SIMD waiting on one or two waves' dependent fetches;
+// spreading them keeps the same rays in flight with less divergence per
+// wave and more waves to hide latency.  It changes which lane runs a ray,
+// never the ray's own sequence of fetches and tests.
+__device__ __forceinline__ int wave_cap(int n, int block_size) {
+    long long waves = (long long)gridDim.x * (block_size / 64);
+    long long c = (n + waves - 1) / waves;
+    return c >= 64 ? 64 : (c < 1 ? 1 : (int)c);
+}
+
+// Hands queue slots to idle lanes: the wave takes up to (cap - busy lanes)
+// consecutive slots from `fetch` with one atomic, in lane order.  Returns
+// this lane's slot (>= n when it gets none) and sets `exhausted` once the
+// queue has been handed out.
+__device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool idle, bool& exhausted) {
+    unsigned long long m = __ballot(idle);
+    int lane = threadIdx.x & 63;
+    int n_idle = __popcll(m);
+    int want = cap - (64 - n_idle);
+    if (want > n_idle) want = n_idle;
+    if (want <= 0) return n;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(fetch, want);
+    base = __shfl(base, 0, 64);
+    if (base + want >= n) exhausted = true;
+    unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    int rank = __popcll(below);
+    return (idle && rank < want) ? base + rank : n;
+}
+
 // ---------------------------------------------------------------- pass control
 __global__ void k_set_job(rx_job* jobs, rx_job j, int32_t* path_off, int32_t* pix_off, int32_t paths, int32_t pix) {
     jobs[0] = j;
@@ -282,6 +316,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                                    (int)gridDim.x * SBS};
     const int n = w.counts[qi];
     int32_t* fetch = &w.counts[3];
+    const int cap = wave_cap(n, SBS);
     // the queue this bounce's shade fills and the next-event queue start empty
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         w.counts[1 - qi] = 0;
@@ -302,15 +337,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     bool have = false;
     while (true) {
         bool idle = i < 0;
-        unsigned long long m = __ballot(idle);
-        if (m != 0ull && !exhausted) {
-            int k = __popcll(m);
-            int base = 0;
-            if (lane == 0) base = atomicAdd(fetch, k);
-            base = __shfl(base, 0, 64);
-            if (base + k >= n) exhausted = true;
-            unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-            int mine = base + __popcll(below);
+        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted);
+        {
             if (idle && mine < n) {
                 i = mine;
                 p = w.q_path[qi][i];
@@ -670,6 +698,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                                    (int)gridDim.x * SBS};
     const int n = w.counts[2];
     int32_t* fetch = &w.counts[4];
+    const int cap = wave_cap(n, SBS);
     const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
     unsigned long long shadow = 0;
@@ -688,15 +717,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     float max_t = 0.0f, ndotl = 0.0f, solid = 0.0f;
     while (true) {
         bool idle = i < 0;
-        unsigned long long m = __ballot(idle);
-        if (m != 0ull && !exhausted) {
-            int k = __popcll(m);
-            int base = 0;
-            if (lane == 0) base = atomicAdd(fetch, k);
-            base = __shfl(base, 0, 64);
-            if (base + k >= n) exhausted = true;
-            unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-            int mine = base + __popcll(below);
+        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted);
+        {
             if (idle && mine < n) {
                 i = mine;
                 p = w.e_path[i];
