@@ -32,6 +32,13 @@ using namespace rx;
 constexpr int BS = 256;
 constexpr int MAX_JOBS = 4096;
 
+// Atomics on one address from every wave of a launch serialise (about
+// 11.5 ns each on MI355X, where device-scope atomics are resolved past the
+// per-XCD L2s); spread over 64 addresses they cost well under 1 ns.  So
+// counts that every wave adds to go to NPART partial slots, picked by
+// blockIdx % NPART, and are folded into their destination once per pass.
+constexpr int NPART = 64;
+
 // Path state and queues, SoA over the workspace.
 struct ws_view {
     rx_job* jobs;
@@ -56,6 +63,12 @@ struct ws_view {
     float *e_px, *e_py, *e_pz, *e_nx, *e_ny, *e_nz, *e_ax, *e_ay, *e_az, *e_tx, *e_ty, *e_tz, *e_scale;
     // queue lengths: [0], [1] ray queues, [2] next-event queue
     int32_t* counts;
+    // per-job queue offsets of the primary rays (n_jobs + 1)
+    int32_t* q_off;
+    // partial sums of the traversal counters and of each job's rays, one
+    // slot per NPART blocks (blockIdx % NPART), folded once per pass
+    unsigned long long* ctr_part;   // NPART x 16 (an rx_counters per 128 bytes)
+    unsigned long long* ray_part;   // NPART x n_jobs
     // traversal stack spill area (entries past the LDS part)
     int2* spill;
 };
@@ -66,6 +79,9 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, ws_view& w) {
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
+    w.q_off = ar.take<int32_t>(n_jobs + 1);
+    w.ctr_part = ar.take<unsigned long long>(NPART * 16);
+    w.ray_part = ar.take<unsigned long long>((size_t)NPART * n_jobs);
     float** pf[] = {&w.thr_x, &w.thr_y, &w.thr_z, &w.col_x, &w.col_y, &w.col_z, &w.ior,
                     &w.fn_x,  &w.fn_y,  &w.fn_z,  &w.fa_x,  &w.fa_y,  &w.fa_z,  &w.fd};
     for (float** f : pf) *f = ar.take<float>(P);
@@ -161,9 +177,13 @@ __device__ __forceinline__ int wave_cap(int n, int block_size) {
 }
 
 // Hands queue slots to idle lanes: the wave takes up to (cap - busy lanes)
-// consecutive slots from `fetch` with one atomic, in lane order.  Returns
+// consecutive slots from `fetch` with one atomic, in lane order.  Every
+// wave of the launch fetches from the same counter, so a full-width wave
+// waits until REFILL_MIN lanes are free (or all are) before it fetches:
+// a few idle lanes cost less than an atomic per finished ray.  Returns
 // this lane's slot (>= n when it gets none) and sets `exhausted` once the
 // queue has been handed out.
+constexpr int REFILL_MIN = 8;
 __device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool idle, bool& exhausted) {
     unsigned long long m = __ballot(idle);
     int lane = threadIdx.x & 63;
@@ -171,6 +191,7 @@ __device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool i
     int want = cap - (64 - n_idle);
     if (want > n_idle) want = n_idle;
     if (want <= 0) return n;
+    if (cap >= 32 && want < REFILL_MIN && n_idle < 64) return n;
     int base = 0;
     if (lane == 0) base = atomicAdd(fetch, want);
     base = __shfl(base, 0, 64);
@@ -178,6 +199,30 @@ __device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool i
     unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
     int rank = __popcll(below);
     return (idle && rank < want) ? base + rank : n;
+}
+
+// Appends to a queue with one atomic per block: every thread of the block
+// calls this at the same point (uniform control flow).  Slots are handed
+// out in thread order within the block.
+__device__ __forceinline__ int block_append(int32_t* counter, bool want, int32_t* lds /* BS / 64 + 1 */) {
+    unsigned long long m = __ballot(want);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) lds[wv] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int k = 0; k < BS / 64; ++k) {
+            int32_t c = lds[k];
+            lds[k] = tot;
+            tot += c;
+        }
+        lds[BS / 64] = tot > 0 ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    int r = lds[BS / 64] + lds[wv] + __popcll(below);
+    __syncthreads();
+    return r;
 }
 
 // ---------------------------------------------------------------- pass control
@@ -189,17 +234,80 @@ __global__ void k_set_job(rx_job* jobs, rx_job j, int32_t* path_off, int32_t* pi
     pix_off[1] = pix;
 }
 
-__global__ void k_pass_begin(int32_t* counts) {
-    if (threadIdx.x < 8) counts[threadIdx.x] = 0;
+// Start of a pass (one block of PB threads): clears the queue lengths and
+// the counter partials, and lays out the primary-ray queue.  Job j's
+// active paths this pass are its first pass_samples * npix path slots, so
+// they get queue slots q_off[j] + local with q_off an exclusive prefix sum
+// over jobs: compaction without atomics, in path order.
+constexpr int PB = 256;
+__global__ __launch_bounds__(PB) void k_pass_begin(ws_view w) {
+    __shared__ int32_t part[PB];
+    const int t = threadIdx.x;
+    if (t < 8) w.counts[t] = 0;
+    for (int k = t; k < NPART * 16; k += PB) w.ctr_part[k] = 0ull;
+    const int per = (w.n_jobs + PB - 1) / PB;
+    const int j0 = t * per, j1 = min(w.n_jobs, j0 + per);
+    int32_t sum = 0;
+    for (int j = j0; j < j1; ++j) {
+        const rx_job& J = w.jobs[j];
+        sum += pass_samples(J) * J.camera.width * J.camera.height;
+    }
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int k = 0; k < PB; ++k) {
+            int32_t c = part[k];
+            part[k] = run;
+            run += c;
+        }
+        w.q_off[w.n_jobs] = run;
+        w.counts[0] = run;
+    }
+    __syncthreads();
+    int32_t run = part[t];
+    for (int j = j0; j < j1; ++j) {
+        w.q_off[j] = run;
+        const rx_job& J = w.jobs[j];
+        run += pass_samples(J) * J.camera.width * J.camera.height;
+    }
 }
 
-// Moves every job's sample window after a pass.
-__global__ void k_advance(rx_job* jobs, int n_jobs) {
+// Clears the per-job ray partials (once, before a job list's first pass;
+// k_advance clears them after every fold).
+__global__ void k_clear_ray_part(ws_view w) {
+    long long n = (long long)NPART * w.n_jobs;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        w.ray_part[k] = 0ull;
+}
+
+// End of a pass: moves every job's sample window, folds the job's ray
+// partials into its count, and (block 0) folds the traversal counter
+// partials into the caller's counters.
+__global__ void k_advance(ws_view w, rx_counters* __restrict__ ctr) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_jobs) return;
-    int ns = pass_samples(jobs[j]);
-    jobs[j].first_sample += ns;
-    jobs[j].samples_left -= ns;
+    if (j < w.n_jobs) {
+        rx_job& J = w.jobs[j];
+        int ns = pass_samples(J);
+        J.first_sample += ns;
+        J.samples_left -= ns;
+        unsigned long long r = 0;
+        for (int k = 0; k < NPART; ++k) {
+            r += w.ray_part[(size_t)k * w.n_jobs + j];
+            w.ray_part[(size_t)k * w.n_jobs + j] = 0ull;
+        }
+        J.rays += r;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 8 && ctr) {
+        unsigned long long sum = 0;
+        for (int k = 0; k < NPART; ++k) sum += w.ctr_part[k * 16 + threadIdx.x];
+        if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(ctr) + threadIdx.x, sum);
+    }
+}
+
+// This block's partial slot of the traversal counters.
+__device__ __forceinline__ rx_counters* ctr_slot(const ws_view& w) {
+    return reinterpret_cast<rx_counters*>(w.ctr_part + (blockIdx.x & (NPART - 1)) * 16);
 }
 
 // ---------------------------------------------------------------- generate
@@ -238,8 +346,8 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 w.p_rays[p] = 0;
             }
         }
-        int slot = wave_append(&w.counts[0], active);
         if (active) {
+            int slot = w.q_off[jx] + (int)(p - w.path_off[jx]);
             w.q_path[0][slot] = (int32_t)p;
             w.q_ox[0][slot] = org.x;
             w.q_oy[0][slot] = org.y;
@@ -277,7 +385,7 @@ __device__ __forceinline__ void closest_prim(const rx_scene& sc, f3 ro, f3 rd, f
 // closest = 1e9 and is compared with it only at the end, intersect.py:292),
 // so computing it first and separately leaves every hit unchanged.
 template <bool COUNT>
-__global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
     const int n = w.counts[qi];
     trav_count cnt{0, 0, 0};
     const int stride = gridDim.x * BS;
@@ -295,7 +403,7 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi, rx_counters
             w.h_id[i] = id;
         }
     }
-    if (COUNT) wave_add(&ctr->prim_tests, cnt.prims);
+    if (COUNT) wave_add(&ctr_slot(w)->prim_tests, cnt.prims);
 }
 
 // ---------------------------------------------------------------- extend
@@ -310,7 +418,7 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi, rx_counters
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
 template <int SBS, bool COUNT, bool PRE>
-__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
@@ -424,6 +532,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
             }
         }
     }
+    rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, rays);
     if (COUNT) {
         wave_add(&ctr->node_pops, cnt.pops);
@@ -485,6 +594,7 @@ __device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
 
 // ---------------------------------------------------------------- shade
 __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
+    __shared__ int32_t app[BS / 64 + 1];
     const int n = w.counts[qi];
     // the next bounce's extend hands out its queue from counts[3], this
     // bounce's connect from counts[4]
@@ -623,7 +733,7 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
             w.col_y[p] = col.y;
             w.col_z[p] = col.z;
         }
-        int slot = wave_append(&w.counts[qo], cont);
+        int slot = block_append(&w.counts[qo], cont, app);
         if (cont) {
             w.q_path[qo][slot] = p;
             w.q_ox[qo][slot] = ro2.x;
@@ -633,7 +743,7 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
             w.q_dy[qo][slot] = rd2.y;
             w.q_dz[qo][slot] = rd2.z;
         }
-        int es = wave_append(&w.counts[2], want_nee);
+        int es = block_append(&w.counts[2], want_nee, app);
         if (want_nee) {
             w.e_path[es] = p;
             w.e_px[es] = hit_p.x;
@@ -692,7 +802,7 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // per-request order of lights and the sum order of `direct` are the
 // reference's.
 template <int SBS, bool COUNT>
-__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAVES_PER_EU))) void k_connect(ws_view w, int depth, rx_counters* __restrict__ ctr) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAVES_PER_EU))) void k_connect(ws_view w, int depth) {
     __shared__ int2 stack_mem[STACK_LDS * SBS];
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
@@ -838,6 +948,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
             }
         }
     }
+    rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, shadow);
     wave_add(&ctr->shadow_rays, shadow);
     if (COUNT) {
@@ -912,14 +1023,16 @@ __global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
                 }
             }
         }
-        // per-job ray totals: one atomic per wave when the wave is one job
+        // per-job ray totals into this block's partial slot (folded by
+        // k_advance): one atomic per wave when the wave is one job
+        unsigned long long* part = w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs;
         int j0 = __shfl(jx, 0, 64);
         bool uniform = __ballot(jx != j0) == 0ull;
         if (uniform) {
             unsigned long long s = wave_sum_u64(rays);
-            if ((threadIdx.x & 63) == 0 && s) atomicAdd(&w.jobs[j0].rays, s);
+            if ((threadIdx.x & 63) == 0 && s) atomicAdd(part + j0, s);
         } else if (rays) {
-            atomicAdd(&w.jobs[jx].rays, rays);
+            atomicAdd(part + jx, rays);
         }
     }
 }
@@ -1011,33 +1124,33 @@ template <int SBS>
 int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
     int gp = grid_for(w.P, BS, 16);
     int gt = trav_grid(w.P, SBS == BS_DEEP);
-    RX_STAGE(tm, 0, (k_pass_begin<<<1, 64, 0, s>>>(w.counts)));
+    RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
         if (pl.prim_prepass) {
             if (pl.count) {
-                RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi, ctr)));
-                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+                RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
+                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(w, qi)));
             } else {
-                RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi, ctr)));
-                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+                RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi)));
+                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(w, qi)));
             }
         } else if (pl.count) {
-            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(w, qi)));
         } else {
-            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(w, qi, ctr)));
+            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(w, qi)));
         }
         RX_STAGE(tm, 2, (k_shade<<<gp, BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             if (pl.count)
-                RX_STAGE(tm, 3, (k_connect<SBS, true><<<gt, SBS, 0, s>>>(w, d, ctr)));
+                RX_STAGE(tm, 3, (k_connect<SBS, true><<<gt, SBS, 0, s>>>(w, d)));
             else
-                RX_STAGE(tm, 3, (k_connect<SBS, false><<<gt, SBS, 0, s>>>(w, d, ctr)));
+                RX_STAGE(tm, 3, (k_connect<SBS, false><<<gt, SBS, 0, s>>>(w, d)));
         }
     }
     RX_STAGE(tm, 4, (k_accumulate<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
-    RX_STAGE(tm, 4, (k_advance<<<(w.n_jobs + 255) / 256, 256, 0, s>>>(w.jobs, w.n_jobs)));
+    RX_STAGE(tm, 4, (k_advance<<<(w.n_jobs + 255) / 256, 256, 0, s>>>(w, ctr)));
     return RX_OK;
 }
 
@@ -1173,6 +1286,8 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
     RX_CHECK_LAUNCH();
+    k_clear_ray_part<<<1, 64, 0, s>>>(w);
+    RX_CHECK_LAUNCH();
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
     tm.s = s;
@@ -1222,6 +1337,11 @@ extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace
         hipError_t e = hipMemcpy(w.jobs, hj, sizeof(rx_job) * n_jobs, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w.path_off, poff, sizeof(int32_t) * (n_jobs + 1), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w.pix_off, qoff, sizeof(int32_t) * (n_jobs + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            k_clear_ray_part<<<grid_for((long long)NPART * n_jobs, 256, 4), 256, 0, nullptr>>>(w);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) st = rx::set_error(RX_ERR_HIP, "job upload failed: %s", hipGetErrorString(e));
     }
     free(poff);
