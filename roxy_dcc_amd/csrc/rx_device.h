@@ -24,6 +24,7 @@
 #define RX_LDS __attribute__((address_space(3)))
 typedef float gv4f __attribute__((ext_vector_type(4)));
 typedef int gv4i __attribute__((ext_vector_type(4)));
+typedef int gv2i __attribute__((ext_vector_type(2)));
 
 // Loads a scene record (a multiple of 16 bytes, 16-byte aligned: the entry
 // points check the array alignment) as 16-byte global loads into a local
@@ -362,9 +363,10 @@ struct lane_stack {
 template <class Stack>
 __device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
                                                float limit, Stack& stk, int& sp, int& next) {
+    // the two child boxes and the two links: 56 of the record's 64 bytes
     const RX_GLOBAL gv4f* p = (const RX_GLOBAL gv4f*)(nodes + link);
     gv4f a = p[0], b = p[1], c = p[2];
-    gv4i d = ((const RX_GLOBAL gv4i*)p)[3];
+    gv2i d = ((const RX_GLOBAL gv2i*)p)[6];
     float lt, rt;
     bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
     bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
@@ -404,11 +406,13 @@ __device__ __forceinline__ bool pop_next(const Stack& stk, int& sp, float limit,
 }
 
 __device__ __forceinline__ void load_tri(const rx_tri* __restrict__ tris, int i, f3& v0, f3& e1, f3& e2) {
+    // v0, e1, e2: the first 36 of the record's 48 bytes
     const RX_GLOBAL gv4f* tp = (const RX_GLOBAL gv4f*)(tris + i);
-    gv4f p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    gv4f p0 = tp[0], p1 = tp[1];
+    float p2 = ((const RX_GLOBAL float*)tp)[8];
     v0 = mk(p0.x, p0.y, p0.z);
     e1 = mk(p0.w, p1.x, p1.y);
-    e2 = mk(p1.z, p1.w, p2.x);
+    e2 = mk(p1.z, p1.w, p2);
 }
 
 // Closest hit over the BVH: rendering/taichi/intersect.py:81-152.  Visit

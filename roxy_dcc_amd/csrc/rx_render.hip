@@ -162,6 +162,28 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     return base + __popcll(below);
 }
 
+// End of the interior phase of a traversal step.  Lanes walk interior
+// nodes until they hold a leaf; the wave then intersects its lanes'
+// leaves together.  Waiting for the last lane to find its leaf issues
+// wide node fetches for a handful of lanes, and the vector-memory return
+// path is paid per instruction, so the phase ends once no more than
+// INNER_EXIT_NUM / INNER_EXIT_DEN of the lanes that entered it are still
+// walking (after at least one step, so every lane progresses); those
+// lanes carry on in the next step.  A lane's own sequence of pops and
+// tests is unchanged.  Measured on MI355X (70k- and 2M-triangle scenes,
+// Mrays/s): running to the last lane 817 / 353, 2/3 827 / 378, 1/2
+// 874 / 395, 3/8 885 / 399, 1/3 880 / 393, 1/4 880 / 391, 1/8 855 / 372.
+#ifndef INNER_EXIT_NUM
+#define INNER_EXIT_NUM 3
+#endif
+#ifndef INNER_EXIT_DEN
+#define INNER_EXIT_DEN 8
+#endif
+__device__ __forceinline__ bool inner_done(int n_in, int entered, int it) {
+    if (n_in == 0) return true;
+    return it > 0 && n_in * INNER_EXIT_DEN <= entered * INNER_EXIT_NUM;
+}
+
 // Rays a wave of a persistent kernel keeps in flight at once: all 64 lanes
 // when the queue has work for every resident wave, otherwise an even share
 // (n / waves, at least 1).  When few rays are left (the later bounces),
@@ -485,12 +507,18 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
         if (__ballot(i >= 0) == 0ull) break;
         if (i >= 0) {
             if (have) {
-                while (have && cur >= 0) {
-                    if (COUNT) cnt.pops++;
-                    if (!visit_interior(nodes, cur, ro, inv, best_t, stk, sp, cur))
-                        have = pop_next(stk, sp, best_t, cur);
+                const int entered = __popcll(__ballot(true));
+                for (int it = 0;; ++it) {
+                    bool in = have && cur >= 0;
+                    int n_in = __popcll(__ballot(in));
+                    if (inner_done(n_in, entered, it)) break;
+                    if (in) {
+                        if (COUNT) cnt.pops++;
+                        if (!visit_interior(nodes, cur, ro, inv, best_t, stk, sp, cur))
+                            have = pop_next(stk, sp, best_t, cur);
+                    }
                 }
-                if (have) {
+                if (have && cur < 0) {
                     if (COUNT) cnt.pops++;
                     int lw = ~cur;
                     int start = lw >> 8, end = start + (lw & 255);
@@ -903,12 +931,18 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
             if (tracing && have && !occ) {
                 const rx_node* nodes = J->scene.nodes;
                 const rx_tri* tris = J->scene.tris;
-                while (have && cur >= 0) {
-                    if (COUNT) cnt.pops++;
-                    if (!visit_interior(nodes, cur, so, inv, max_t, stk, sp, cur))
-                        have = pop_next(stk, sp, max_t, cur);
+                const int entered = __popcll(__ballot(true));
+                for (int it = 0;; ++it) {
+                    bool in = have && cur >= 0;
+                    int n_in = __popcll(__ballot(in));
+                    if (inner_done(n_in, entered, it)) break;
+                    if (in) {
+                        if (COUNT) cnt.pops++;
+                        if (!visit_interior(nodes, cur, so, inv, max_t, stk, sp, cur))
+                            have = pop_next(stk, sp, max_t, cur);
+                    }
                 }
-                if (have) {
+                if (have && cur < 0) {
                     if (COUNT) cnt.pops++;
                     int lw = ~cur;
                     int start = lw >> 8, end = start + (lw & 255);
