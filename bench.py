@@ -19,6 +19,7 @@ Prints one JSON line:
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import ctypes as C
 import json
 import os
@@ -186,6 +187,168 @@ def large_scene_extras(steps=2, extra_flags=0):
             "traversal_work_per_render": work}
 
 
+def _plain_render(R, torch, world, W, H, st, sc, seed, per_sample=False):
+    """One request through the single-job path, then the frame read back.
+    per_sample: the reference renderer's execution model, one launch
+    sequence per sample (rendering/taichi_renderer.py:140-156); otherwise
+    the drop-in's policy of passes of up to 2M paths."""
+    spp = 1 if per_sample else R.default_samples_per_pass(W, H, st["samples"])
+    integ = R.Integrator(sc, W, H, spp)
+    frame = R.DeviceFrame(W, H)
+    cam = R.camera_struct(world.active_camera, W, H)
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                          world._background_color, seed, spp)
+    done = 0
+    while done < st["samples"]:
+        n = min(spp, st["samples"] - done)
+        integ.render(frame, cam, prm, done, n)
+        done += n
+    return frame.color.cpu().numpy(), integ.counts()["rays"]
+
+
+def batching_bench(n_scenes=256, seed0=4000):
+    """Many uneven scenes (displaced spheres of 1k..200k triangles, 64^2..512^2,
+    4..32 spp, depth 6) rendered one request at a time through the plain path,
+    then as one autotuned batch (one set of launches per pass for all jobs,
+    replayed from a HIP graph).  Scenes are resident on both sides; both
+    include reading every image back.  Images must be bit-identical, so
+    the two are compared at equal RMSE by construction."""
+    import torch
+
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.autotune import Autotuner
+    from roxy_dcc_amd.batch import Batch, RenderJob
+    from roxy_dcc_amd.extract import extract_scene
+
+    t0 = time.perf_counter()
+    items = []
+    for i in range(n_scenes):
+        world, W, H, st = scenes.batch_scene(seed0 + i)
+        items.append((world, W, H, st, extract_scene(world).scene))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    tris = sum(it[4].n_tris for it in items)
+    pixels = sum(it[1] * it[2] for it in items)
+    samples = sum(it[1] * it[2] * it[3]["samples"] for it in items)
+
+    _plain_render(R, torch, *items[0][:5], seed0)          # warm-up
+    plain, plain_s, plain_rays = {}, {}, {}
+    for mode in ("per_sample", "drop_in"):
+        t0 = time.perf_counter()
+        imgs, rays = [], 0
+        for i, (world, W, H, st, sc) in enumerate(items):
+            img, r = _plain_render(R, torch, world, W, H, st, sc, seed0 + i, per_sample=mode == "per_sample")
+            imgs.append(img)
+            rays += r
+        plain_s[mode] = time.perf_counter() - t0
+        plain[mode], plain_rays[mode] = imgs, rays
+
+    jobs = [RenderJob(sc, world.active_camera, W, H, st["samples"], max_depth=st["max_depth"],
+                      direct_light_mode=st["direct_light_mode"], direct_light_max_depth=st["direct_light_max_depth"],
+                      use_sky=bool(world._use_sky), background=tuple(world._background_color), seed=seed0 + i)
+            for i, (world, W, H, st, sc) in enumerate(items)]
+    tuner = Autotuner()
+    t0 = time.perf_counter()
+    tuning = tuner.tune(jobs)
+    tune_s = time.perf_counter() - t0
+    tuning.apply(jobs)
+    t0 = time.perf_counter()
+    b = Batch(jobs, use_graph=True)
+    b.run()
+    torch.cuda.synchronize()
+    batched = [j.frame.color.cpu().numpy() for j in jobs]
+    batch_s = time.perf_counter() - t0
+    rays = b.rays()
+    b.close()
+    same = {m: sum(int(np.array_equal(a, c)) for a, c in zip(plain[m], batched)) for m in plain}
+    return {"workload": f"{n_scenes} scenes (seeds {seed0}..{seed0 + n_scenes - 1}): displaced UV spheres of 1k..200k "
+                        "triangles in the Cornell enclosure, 64^2..512^2, 4..32 spp, depth 6",
+            "triangles": tris, "pixels": pixels, "samples": samples, "scene_setup_s": setup,
+            "batched_s": batch_s, "batched_renders_per_s": n_scenes / batch_s,
+            "batched_Mrays_per_s": rays / batch_s / 1e6,
+            "plain_per_sample_s": plain_s["per_sample"],
+            "plain_per_sample_renders_per_s": n_scenes / plain_s["per_sample"],
+            "speedup_vs_plain_per_sample": plain_s["per_sample"] / batch_s,
+            "plain_drop_in_s": plain_s["drop_in"], "plain_drop_in_renders_per_s": n_scenes / plain_s["drop_in"],
+            "plain_drop_in_Mrays_per_s": plain_rays["drop_in"] / plain_s["drop_in"] / 1e6,
+            "speedup_vs_plain_drop_in": plain_s["drop_in"] / batch_s,
+            "identical_images": same, "tuning": dataclasses.asdict(tuning), "tune_s": tune_s,
+            "tune_trials": [[dataclasses.asdict(t), r / 1e6] for t, r in tuner.log]}
+
+
+def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
+    """Poisson arrivals of render requests drawn from the mesh (70k
+    triangles) and large (2M triangles) scene generators, 128^2..1080p,
+    4..64 spp, served by the continuous-batching server and by the
+    one-request-at-a-time loop.  Offered load is a multiple of the serial
+    loop's capacity (requests/s it sustains with every request queued at
+    t = 0).  Each request's image must be the same in both."""
+    import torch
+
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.extract import extract_scene
+    from roxy_dcc_amd.serve import RenderServer, Request, SceneCache, latency_stats, poisson_arrivals, serial_loop
+
+    def load(key):
+        kind, sd = key
+        world, *_ = scenes.mesh_scene(seed=sd) if kind == "mesh" else scenes.large_scene(seed=sd)
+        return extract_scene(world).scene, world.active_camera
+
+    rng = np.random.default_rng(seed)
+    keys = [("mesh", 20 + k) for k in range(6)] + [("large", 30 + k) for k in range(2)]
+    sizes = [(128, 128), (256, 256), (512, 512), (960, 540), (1920, 1080)]
+    spec = []
+    for i in range(n_requests):
+        key = keys[int(rng.integers(len(keys)))]
+        W, H = sizes[int(rng.choice(len(sizes), p=[0.3, 0.3, 0.2, 0.15, 0.05]))]
+        spp = int(rng.choice([4, 8, 16, 32, 64]))
+        spec.append((key, W, H, spp))
+
+    def requests(arrivals):
+        return [Request(i, key, W, H, spp, float(a), max_depth=6, seed=1000 + i)
+                for i, ((key, W, H, spp), a) in enumerate(zip(spec, arrivals))]
+
+    cache = SceneCache(load, max_scenes=32)
+    t0 = time.perf_counter()
+    for k in keys:
+        cache.get(k)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    serial_loop(cache, requests([0.0] * 2)[:2])               # warm-up
+    t0 = time.perf_counter()
+    ref = serial_loop(cache, requests([0.0] * n_requests))
+    cap = n_requests / (time.perf_counter() - t0)
+    ref_frames = {r.rid: r.frame for r in ref}
+    rows = []
+    for f in loads:
+        rate = f * cap
+        arr = poisson_arrivals(rate, n_requests, seed=seed)
+        row = {"offered_load": f, "rate_per_s": rate}
+        t0 = time.perf_counter()
+        done = serial_loop(cache, requests(arr))
+        row["serial"] = latency_stats(done, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        done = serial_loop(cache, requests(arr), paths_per_pass=1)
+        row["serial_per_sample"] = latency_stats(done, time.perf_counter() - t0)
+        same = 0
+        for policy in ("fair", "fifo"):
+            server = RenderServer(cache, max_jobs=32, policy=policy)
+            t0 = time.perf_counter()
+            done = server.run(requests(arr))
+            row["server_" + policy] = dict(latency_stats(done, time.perf_counter() - t0), passes=server.passes)
+            same += sum(int(np.array_equal(r.frame, ref_frames[r.rid])) for r in done)
+        row["identical_images"] = same
+        rows.append(row)
+    return {"workload": f"{n_requests} requests over {len(keys)} resident scenes (6 mesh, 2 large), "
+                        "128^2..1920x1080, 4..64 spp, depth 6, Poisson arrivals; offered load relative to the "
+                        "serial loop's capacity; serial = passes of up to 4M paths, serial_per_sample = one "
+                        "sample per pass (the reference's launch-per-sample model); identical_images counts "
+                        "both server policies against the serial images",
+            "scene_setup_s": setup, "serial_capacity_per_s": cap, "sweep": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +359,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-pass", type=int, default=None)
     ap.add_argument("--no-large", action="store_true", help="skip the 2M-triangle 1080p scene extras")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batching comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) comparison")
     ap.add_argument("--prim-prepass", action="store_true",
                     help="run the closest-primitive scan as its own pass (RX_FLAG_PRIM_PREPASS)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -304,6 +469,10 @@ def main():
     }
     if not args.no_large and args.config != "large":
         out["large_scene"] = large_scene_extras(extra_flags=extra_flags)
+    if not args.no_batch:
+        out["batching"] = batching_bench()
+    if not args.no_serve:
+        out["serving"] = serving_bench()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -182,6 +182,10 @@ typedef struct rx_camera {
 #define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
 #define RX_FLAG_PRIM_PREPASS 4   /* closest-primitive scan as its own pass over each ray
                                     queue (automatic above 256 analytic primitives) */
+#define RX_FLAG_SMALL_BLOCKS 8   /* traversal in 64-thread blocks with the deep stack (the
+                                    shape used anyway for BVHs deeper than 23).  The batch
+                                    workspace query sees it; for rx_render size the
+                                    workspace with bvh_depth = RX_MAX_BVH_DEPTH. */
 
 /* Integrator settings (render_kernel arguments, trace.py:239-254). */
 typedef struct rx_render_params {
@@ -195,6 +199,9 @@ typedef struct rx_render_params {
     uint32_t seed_hi;
     int32_t flags;
     int32_t samples_per_pass;        /* samples traced together per wavefront pass */
+    int32_t trav_blocks_per_cu;      /* resident blocks per CU of the persistent traversal
+                                        kernels: 0 = the register budget's maximum (8 blocks
+                                        of 256, or 32 of 64 threads), else 1 .. that maximum */
 } rx_render_params;
 
 /* Per-pixel running means written by rx_render (fields.py:14-19). */

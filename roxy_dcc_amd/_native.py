@@ -30,6 +30,7 @@ RX_MAX_BVH_DEPTH = 63
 RX_FLAG_PROFILE = 1
 RX_FLAG_COUNT_TRAFFIC = 2
 RX_FLAG_PRIM_PREPASS = 4
+RX_FLAG_SMALL_BLOCKS = 8
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -63,7 +64,7 @@ class RxRenderParams(C.Structure):
     _fields_ = [
         ("max_depth", i32), ("use_sky", i32), ("direct_light_mode", i32), ("direct_light_max_depth", i32),
         ("sample_clamp", C.c_float), ("bg", C.c_float * 3), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32),
-        ("flags", i32), ("samples_per_pass", i32),
+        ("flags", i32), ("samples_per_pass", i32), ("trav_blocks_per_cu", i32),
     ]
 
 

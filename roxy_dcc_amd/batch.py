@@ -32,6 +32,8 @@ class RenderJob:
     background: tuple = (0.0, 0.0, 0.0)
     seed: int = 0
     samples_per_pass: int = 1
+    trav_blocks_per_cu: int = 0     # launch shape (see autotune.Tuning)
+    flags: int = 0
     first_sample: int = 0
     frame: DeviceFrame = field(default=None, repr=False)
     tag: object = None
@@ -44,7 +46,8 @@ class RenderJob:
         j.scene = self.scene.struct()
         j.camera = camera_struct(self.camera, self.width, self.height)
         j.params = params_struct(self.max_depth, int(self.use_sky), mode, self.direct_light_max_depth,
-                                 self.sample_clamp, self.background, self.seed, self.samples_per_pass, flags)
+                                 self.sample_clamp, self.background, self.seed, self.samples_per_pass,
+                                 flags | self.flags, self.trav_blocks_per_cu)
         j.accum = self.frame.struct()
         j.first_sample = self.first_sample
         j.samples_left = self.samples - self.first_sample

@@ -53,8 +53,13 @@ inline bool use_deep_stack(int depth) { return depth + 1 > STACK_SHALLOW; }
 
 // Grid of a traversal kernel over up to n rays, and the spill area (int2
 // entries) that grid needs.
-inline int trav_grid(long long n, bool deep) {
-    return deep ? grid_for(n, BS_DEEP, TRAV_BLOCKS_PER_CU_DEEP) : grid_for(n, BS_SHALLOW, TRAV_BLOCKS_PER_CU_SHALLOW);
+// per_cu = 0 takes the default (the most blocks the register budget keeps
+// resident); a tuned value is clamped to 1 .. that default, which is what
+// the spill area is sized for.
+inline int trav_grid(long long n, bool deep, int per_cu = 0) {
+    int lim = deep ? TRAV_BLOCKS_PER_CU_DEEP : TRAV_BLOCKS_PER_CU_SHALLOW;
+    int pc = per_cu <= 0 || per_cu > lim ? lim : per_cu;
+    return grid_for(n, deep ? BS_DEEP : BS_SHALLOW, pc);
 }
 inline size_t trav_spill_entries(bool deep) {
     long long threads = deep ? (long long)device_cu_count() * TRAV_BLOCKS_PER_CU_DEEP * BS_DEEP

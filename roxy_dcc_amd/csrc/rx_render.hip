@@ -1143,6 +1143,7 @@ struct pass_plan {
     bool deep;               // stack class
     bool count;              // traversal counters
     bool prim_prepass;       // closest-primitive scan as its own pass
+    int trav_per_cu;         // resident traversal blocks per CU (0 = default)
 };
 
 // Scenes with more analytic primitives than this get their closest-
@@ -1157,7 +1158,7 @@ constexpr int PRIM_INLINE_MAX = 256;
 template <int SBS>
 int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
     int gp = grid_for(w.P, BS, 16);
-    int gt = trav_grid(w.P, SBS == BS_DEEP);
+    int gt = trav_grid(w.P, SBS == BS_DEEP, pl.trav_per_cu);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     for (int d = 0; d < pl.max_depth; ++d) {
@@ -1201,6 +1202,8 @@ int check_job(const rx_job& j, int idx) {
     RX_REQUIRE(j.first_sample >= 0 && j.samples_left >= 0, RX_ERR_INVALID, "job %d: negative sample range", idx);
     RX_REQUIRE(p.samples_per_pass > 0, RX_ERR_INVALID, "job %d: samples_per_pass must be positive", idx);
     RX_REQUIRE(p.max_depth >= 0, RX_ERR_INVALID, "job %d: max_depth must be >= 0", idx);
+    RX_REQUIRE(p.trav_blocks_per_cu >= 0 && p.trav_blocks_per_cu <= TRAV_BLOCKS_PER_CU_DEEP, RX_ERR_INVALID,
+               "job %d: trav_blocks_per_cu must be in 0..%d", idx, TRAV_BLOCKS_PER_CU_DEEP);
     RX_REQUIRE(p.direct_light_mode == 0 || p.direct_light_mode == 1, RX_ERR_INVALID,
                "direct_light_mode must be 'one' or 'all'");
     RX_REQUIRE(j.accum.color && j.accum.normal && j.accum.albedo && j.accum.depth, RX_ERR_INVALID,
@@ -1225,7 +1228,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false, false};
+    pass_plan plan{0, 0, false, false, false, 0};
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -1239,6 +1242,8 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
         if (j.scene.n_lights > 0 && j.params.direct_light_max_depth > plan.nee_depth)
             plan.nee_depth = j.params.direct_light_max_depth;
         if (j.scene.n_tris > 0 && use_deep_stack(j.scene.bvh_depth)) plan.deep = true;
+        if (j.params.flags & RX_FLAG_SMALL_BLOCKS) plan.deep = true;
+        if (j.params.trav_blocks_per_cu > plan.trav_per_cu) plan.trav_per_cu = j.params.trav_blocks_per_cu;
         if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
         if (j.scene.n_prims > 0 && (j.scene.n_prims > PRIM_INLINE_MAX || (j.params.flags & RX_FLAG_PRIM_PREPASS)))
             plan.prim_prepass = true;

@@ -99,11 +99,12 @@ def camera_struct(cam, width, height):
                         fov_tan, float(cam.aspect_ratio), int(width), int(height))
 
 
-def params_struct(max_depth, use_sky, mode_id, dl_max_depth, sample_clamp, bg, seed, samples_per_pass, flags=0):
+def params_struct(max_depth, use_sky, mode_id, dl_max_depth, sample_clamp, bg, seed, samples_per_pass, flags=0,
+                  trav_blocks_per_cu=0):
     seed = int(seed) & ((1 << 64) - 1)
     return nat.RxRenderParams(int(max_depth), int(use_sky), int(mode_id), int(dl_max_depth), float(sample_clamp),
                               (C.c_float * 3)(*[float(x) for x in bg]), seed & 0xFFFFFFFF, seed >> 32, int(flags),
-                              int(samples_per_pass))
+                              int(samples_per_pass), int(trav_blocks_per_cu))
 
 
 class TaichiRenderer:

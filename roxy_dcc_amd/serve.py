@@ -3,13 +3,15 @@
 Requests (scene, resolution, samples) arrive over time.  The server keeps
 up to ``max_scenes`` scene BVHs resident (least recently used evicted,
 never one an active job needs), admits waiting requests in arrival order
-while the job-slot and HBM budgets allow, and every iteration traces one
-wavefront pass of *all* active jobs together (``Batch``).  Jobs that reach
-their sample target are retired and their frames read back; new arrivals
-join at the next iteration, so their work overlaps with jobs already on the
-GPU.  A job's image is the same alone and in any mix (sampling is keyed by
-job seed, pixel, sample and bounce; accumulation is per job in sample
-order).
+while job slots and the frame-memory budget allow, and every iteration
+traces one wavefront pass of *all* active jobs together (``Batch``).  The
+pass's path budget is shared out by water-filling (``share_pass``), so the
+device stays full while small jobs finish in a pass or two and large ones
+take what is left.  Finished jobs are retired and their frames read back;
+new arrivals join at the next iteration, so their work overlaps with jobs
+already on the GPU.  A job's image is the same alone and in any mix
+(sampling is keyed by job seed, pixel, sample and bounce; accumulation is
+per job in sample order), whatever samples per pass it is given.
 
 ``serial_loop`` is the one-job-at-a-time baseline: each request is rendered
 to completion, in arrival order, before the next starts.
@@ -79,48 +81,79 @@ class SceneCache:
             self.evictions += 1
 
 
-def _paths(req, spp_pass):
-    return req.width * req.height * spp_pass
+def share_pass(demands, budget, policy="fair"):
+    """Split of a pass's path budget: `demands` are (pixels, samples_left)
+    per job in arrival order; returns samples per pass for each (whole
+    samples, at least one, at most what the job still needs).
+
+    "fair": water-filling, smallest remaining work first, each job taking
+    at most an equal share of what is left, so small jobs finish in one
+    pass and the budget they leave goes to the larger ones (processor
+    sharing: low median latency).
+    "fifo": jobs take the budget in arrival order, as much as each can use,
+    and every other job still advances one sample per pass (first come
+    first served with backfill: bounds the longest wait)."""
+    n = len(demands)
+    out = [1] * n
+    if policy == "fifo":
+        left = budget - sum(px for px, _ in demands)
+        for k, (px, need) in enumerate(demands):
+            extra = max(0, min(need - 1, left // max(1, px)))
+            out[k] = 1 + extra
+            left -= extra * px
+        return out
+    order = sorted(range(n), key=lambda k: demands[k][0] * demands[k][1])
+    left = budget
+    for rank, k in enumerate(order):
+        px, need = demands[k]
+        share = left // (n - rank)
+        spp = max(1, min(need, share // max(1, px)))
+        out[k] = spp
+        left = max(0, left - spp * px)
+    return out
 
 
 class RenderServer:
-    def __init__(self, cache: SceneCache, max_jobs=32, paths_per_pass=1 << 22, job_bytes_budget=32 << 30):
+    """Continuous batching: up to ``max_jobs`` requests share every pass.
+
+    Each iteration traces one wavefront pass of all active jobs together.
+    The pass's path budget (``pass_paths``, enough to fill the GPU) is
+    split over the active jobs by ``share_pass``: processor sharing with
+    the device kept full.  Requests are admitted in arrival order while job
+    slots and the frame-memory budget allow."""
+
+    def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=1 << 23, frame_bytes_budget=32 << 30,
+                 policy="fair"):
         self.cache = cache
+        self.policy = policy
         self.max_jobs = max_jobs
-        self.paths_per_pass = paths_per_pass
-        self.job_bytes_budget = job_bytes_budget
+        self.pass_paths = pass_paths
+        self.frame_bytes_budget = frame_bytes_budget
         self.waiting = collections.deque()
         self.active = []       # (Request, RenderJob)
         self.done = []
         self.passes = 0
 
-    def _spp_pass(self, req):
-        return int(max(1, min(req.samples, self.paths_per_pass // max(1, req.width * req.height * 4))))
-
     @staticmethod
-    def _job_bytes(req, spp_pass):
-        # accumulators (13 floats/pixel) + path state and queues (~47 words/path)
-        return req.width * req.height * 13 * 4 + _paths(req, spp_pass) * 188
+    def _frame_bytes(req):
+        # colour, normal, albedo, display (3 floats) and depth per pixel
+        return req.width * req.height * 13 * 4
 
     def _admit(self, now):
-        used_paths = sum(_paths(r, j.samples_per_pass) for r, j in self.active)
-        used_bytes = sum(self._job_bytes(r, j.samples_per_pass) for r, j in self.active)
+        used = sum(self._frame_bytes(r) for r, _ in self.active)
         while self.waiting and self.waiting[0].arrival <= now and len(self.active) < self.max_jobs:
             req = self.waiting[0]
-            spp = self._spp_pass(req)
-            need = self._job_bytes(req, spp)
-            if self.active and (used_paths + _paths(req, spp) > self.paths_per_pass * 2
-                                or used_bytes + need > self.job_bytes_budget):
+            need = self._frame_bytes(req)
+            if self.active and used + need > self.frame_bytes_budget:
                 break
             self.waiting.popleft()
             pinned = [r.scene_key for r, _ in self.active]
             scene, cam = self.cache.get(req.scene_key, pinned=pinned)
             job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
-                            samples_per_pass=spp, tag=req)
+                            samples_per_pass=1, tag=req)
             req.admitted = now
             self.active.append((req, job))
-            used_paths += _paths(req, spp)
-            used_bytes += need
+            used += need
 
     def run(self, requests, clock=time.perf_counter, idle_sleep=0.0005):
         """Serve ``requests`` (sorted by arrival); returns the finished list."""
@@ -135,12 +168,15 @@ class RenderServer:
                 time.sleep(idle_sleep)
                 continue
             jobs = [j for _, j in self.active]
+            spp = share_pass([(j.width * j.height, j.samples - j.first_sample) for j in jobs], self.pass_paths,
+                             self.policy)
+            for j, k in zip(jobs, spp):
+                j.samples_per_pass = k
             batch = Batch(jobs, use_graph=False)
             batch.run(1)
             recs = batch.read_jobs()
             batch.close()
             self.passes += 1
-            now = clock() - t0
             still = []
             for (req, job), rec in zip(self.active, recs):
                 job.first_sample = rec.first_sample
@@ -157,7 +193,9 @@ class RenderServer:
 
 
 def serial_loop(cache: SceneCache, requests, paths_per_pass=1 << 22, clock=time.perf_counter):
-    """One request at a time, in arrival order, each rendered to completion."""
+    """One request at a time, in arrival order, each rendered to completion
+    in passes of up to `paths_per_pass` paths (1 = one sample per pass, the
+    reference renderer's launch-per-sample model)."""
     import torch
 
     t0 = clock()

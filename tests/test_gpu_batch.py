@@ -80,3 +80,36 @@ def test_uneven_jobs_and_oracle_parity():
         print(f"job {j.seed} {j.width}x{j.height}x{j.samples}: exact {same:.4f}", flush=True)
         assert same >= 0.99
     b.close()
+
+
+def test_autotuned_shapes_leave_images_unchanged():
+    """Every launch shape the autotuner can pick (queue size, traversal
+    occupancy, 64-thread blocks, primitive pre-pass) gives the same images."""
+    import dataclasses
+
+    import torch
+
+    from roxy_dcc_amd.autotune import Autotuner, Tuning
+    from roxy_dcc_amd.batch import Batch
+
+    seeds = [4020, 4021, 4022]
+    ref = _jobs(seeds, res_override=40)
+    Batch(ref, use_graph=False).run()
+    torch.cuda.synchronize()
+    shapes = [Tuning(passes=1), Tuning(passes=3, blocks_per_cu=4), Tuning(passes=2, small_blocks=True),
+              Tuning(passes=6, blocks_per_cu=2, prim_prepass=True)]
+    for t in shapes:
+        jobs = t.apply(_jobs(seeds, res_override=40))
+        b = Batch(jobs, use_graph=True)
+        b.run()
+        torch.cuda.synchronize()
+        for r, j in zip(ref, jobs):
+            assert np.array_equal(_color(r), _color(j)), (t, j.seed)
+        b.close()
+    tuner = Autotuner()
+    jobs = _jobs(seeds, res_override=40)
+    best = tuner.tune(jobs)
+    print(f"tuned: {best}; trials {len(tuner.log)}", flush=True)
+    assert isinstance(best, Tuning) and len(tuner.log) >= 3
+    assert tuner.tune(jobs) is best            # cached by signature
+    assert dataclasses.asdict(best)["passes"] >= 1

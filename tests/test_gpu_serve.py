@@ -30,7 +30,7 @@ def _requests():
 def test_served_results_equal_alone_results():
     from roxy_dcc_amd.serve import RenderServer, SceneCache, latency_stats, serial_loop
 
-    server = RenderServer(SceneCache(_loader(), max_scenes=3), max_jobs=4, paths_per_pass=4096)
+    server = RenderServer(SceneCache(_loader(), max_scenes=3), max_jobs=4, pass_paths=4096)
     done = server.run(_requests())
     assert len(done) == 6 and server.passes > 1
     alone = {r.rid: r for r in serial_loop(SceneCache(_loader()), _requests(), paths_per_pass=1 << 20)}

@@ -290,3 +290,48 @@ def test_poisson_arrivals_rate():
     t = poisson_arrivals(50.0, 20000, seed=1)
     assert np.all(np.diff(t) > 0)
     assert abs(len(t) / t[-1] - 50.0) < 2.0
+
+
+def test_autotune_candidates_and_apply():
+    """Pass-count candidates stay within the path and memory budgets; a
+    tuning sets every job's samples per pass and launch shape."""
+    from types import SimpleNamespace
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd.autotune import BYTES_PER_PATH, Tuning, pass_candidates
+
+    jobs = [SimpleNamespace(width=w, height=w, samples=s, samples_per_pass=1, trav_blocks_per_cu=0, flags=0)
+            for w, s in [(64, 4), (512, 32), (256, 17), (128, 9)]]
+    ks = pass_candidates(jobs, budget_bytes=1 << 30)
+    assert ks and all(1 <= k <= 32 for k in ks) and len(set(ks)) == len(ks)
+    for k in ks:
+        per_pass = sum(j.width * j.height * -(-j.samples // k) for j in jobs)
+        assert per_pass * BYTES_PER_PATH <= 1 << 30
+    t = Tuning(passes=4, blocks_per_cu=6, small_blocks=True, prim_prepass=True)
+    t.apply(jobs)
+    assert [j.samples_per_pass for j in jobs] == [1, 8, 5, 3]
+    assert all(j.trav_blocks_per_cu == 6 for j in jobs)
+    assert all(j.flags == nat.RX_FLAG_SMALL_BLOCKS | nat.RX_FLAG_PRIM_PREPASS for j in jobs)
+
+
+def test_share_pass_water_filling():
+    """Pass budget split: every job gets >= 1 sample and never more than it
+    needs; small jobs are served in full; the budget is not exceeded once
+    every job's one-sample minimum fits."""
+    from roxy_dcc_amd.serve import share_pass
+
+    demands = [(64 * 64, 4), (512 * 512, 32), (1920 * 1080, 64), (128 * 128, 16)]
+    spp = share_pass(demands, 1 << 23)
+    assert all(1 <= s <= need for s, (_, need) in zip(spp, demands))
+    assert spp[0] == 4 and spp[3] == 16
+    assert sum(s * px for s, (px, _) in zip(spp, demands)) <= 1 << 23
+    # one job alone takes as much of the budget as it can use
+    assert share_pass([(512 * 512, 64)], 1 << 23) == [32]
+    # a frame larger than the budget still advances one sample per pass
+    assert share_pass([(1920 * 1080, 8)], 1 << 20) == [1]
+    # first come first served with backfill: the oldest job takes what it
+    # can use, the rest advance at least one sample
+    fifo = share_pass(demands, 1 << 23, "fifo")
+    assert all(1 <= s <= need for s, (_, need) in zip(fifo, demands))
+    assert fifo[0] == 4 and fifo[1] == 23 and fifo[2] == 1 and fifo[3] == 16
+    assert sum(s * px for s, (px, _) in zip(fifo, demands)) <= 1 << 23
