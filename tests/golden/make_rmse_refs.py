@@ -27,13 +27,14 @@ from roxy_dcc_amd.extract import flatten_scene  # noqa: E402
 from roxy_dcc_amd.renderer import TaichiRenderer  # noqa: E402
 from tests import helpers  # noqa: E402
 
-N_PIXELS = 4096
+N_PIXELS = {"cornell": 4096, "mesh": 4096, "large": 16384}
 SPP = 4096
 PIXEL_SEED = 1234
 RENDER_SEED = 0x5EED
 
 
-def make(name, n_pixels=N_PIXELS, spp=SPP):
+def make(name, n_pixels=None, spp=SPP):
+    n_pixels = n_pixels or N_PIXELS[name]
     world, W, H, st = scenes.CONFIGS[name]()
     flat = flatten_scene(world)
     osc = helpers.flat_to_oracle(flat)
@@ -54,5 +55,5 @@ def make(name, n_pixels=N_PIXELS, spp=SPP):
 
 
 if __name__ == "__main__":
-    for name in (sys.argv[1:] or ["cornell", "mesh"]):
+    for name in (sys.argv[1:] or ["cornell", "mesh", "large"]):
         make(name)
