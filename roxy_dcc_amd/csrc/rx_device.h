@@ -22,9 +22,27 @@
 // makes the loads global_load_* and the LDS stack ds_*.
 #define RX_GLOBAL __attribute__((address_space(1)))
 #define RX_LDS __attribute__((address_space(3)))
+// Constant address space: loads at a wave-uniform address through it are
+// scalar loads (s_load, scalar cache), which do not use the vector memory
+// path.  Only for data no kernel writes while the reading kernel runs.
+#define RX_CONST __attribute__((address_space(4)))
 typedef float gv4f __attribute__((ext_vector_type(4)));
 typedef int gv4i __attribute__((ext_vector_type(4)));
 typedef int gv2i __attribute__((ext_vector_type(2)));
+
+// Loads a record at a wave-uniform address with scalar loads.
+template <class T>
+__device__ __forceinline__ T cload(const T* p) {
+    static_assert(sizeof(T) % 16 == 0, "scene records are whole 16-byte words");
+    constexpr int N = sizeof(T) / 16;
+    gv4f tmp[N];
+    const RX_CONST gv4f* q = (const RX_CONST gv4f*)p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) tmp[i] = q[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
 
 // Loads a scene record (a multiple of 16 bytes, 16-byte aligned: the entry
 // points check the array alignment) as 16-byte global loads into a local

@@ -51,13 +51,10 @@ struct ws_view {
     float *thr_x, *thr_y, *thr_z, *col_x, *col_y, *col_z, *ior;
     float *fn_x, *fn_y, *fn_z, *fa_x, *fa_y, *fa_z, *fd;
     int32_t *p_job, *p_rays;
-    // two ray queues
-    int32_t* q_path[2];
-    float *q_ox[2], *q_oy[2], *q_oz[2], *q_dx[2], *q_dy[2], *q_dz[2];
-    // hits of the current queue
-    float* h_t;
-    int32_t *h_id, *h_kind;
-    float *h_u, *h_v;
+    // two ray queues: 32 B per slot, {origin, path index bits} {direction, 0}
+    gv4f* q_ray[2];
+    // hits of the current queue: 16 B per slot, {t, coded id bits, u, v}
+    gv4f* h_rec;
     // next-event requests
     int32_t* e_path;
     float *e_px, *e_py, *e_pz, *e_nx, *e_ny, *e_nz, *e_ax, *e_ay, *e_az, *e_tx, *e_ty, *e_tz, *e_scale;
@@ -87,20 +84,8 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, ws_view& w) {
     for (float** f : pf) *f = ar.take<float>(P);
     w.p_job = ar.take<int32_t>(P);
     w.p_rays = ar.take<int32_t>(P);
-    for (int b = 0; b < 2; ++b) {
-        w.q_path[b] = ar.take<int32_t>(P);
-        w.q_ox[b] = ar.take<float>(P);
-        w.q_oy[b] = ar.take<float>(P);
-        w.q_oz[b] = ar.take<float>(P);
-        w.q_dx[b] = ar.take<float>(P);
-        w.q_dy[b] = ar.take<float>(P);
-        w.q_dz[b] = ar.take<float>(P);
-    }
-    w.h_t = ar.take<float>(P);
-    w.h_id = ar.take<int32_t>(P);
-    w.h_kind = ar.take<int32_t>(P);
-    w.h_u = ar.take<float>(P);
-    w.h_v = ar.take<float>(P);
+    for (int b = 0; b < 2; ++b) w.q_ray[b] = ar.take<gv4f>(2 * P);
+    w.h_rec = ar.take<gv4f>(P);
     w.e_path = ar.take<int32_t>(P);
     float** pe[] = {&w.e_px, &w.e_py, &w.e_pz, &w.e_nx, &w.e_ny, &w.e_nz, &w.e_ax,
                     &w.e_ay, &w.e_az, &w.e_tx, &w.e_ty, &w.e_tz, &w.e_scale};
@@ -123,6 +108,46 @@ __device__ __forceinline__ int owner(const int32_t* __restrict__ off, int n, lon
 }
 
 __device__ __forceinline__ int job_of_path(const ws_view& w, int p) { return w.n_jobs == 1 ? 0 : w.p_job[p]; }
+
+// Ray queue records: two 16-byte words per slot, so a ray is one pair of
+// wide loads or stores.
+__device__ __forceinline__ void put_ray(const ws_view& w, int b, int slot, int path, f3 o, f3 d) {
+    RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)slot;
+    q[0] = gv4f{o.x, o.y, o.z, __int_as_float(path)};
+    q[1] = gv4f{d.x, d.y, d.z, 0.0f};
+}
+
+__device__ __forceinline__ int get_ray(const ws_view& w, int b, int slot, f3& o, f3& d) {
+    const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)slot;
+    gv4f a = q[0], c = q[1];
+    o = mk(a.x, a.y, a.z);
+    d = mk(c.x, c.y, c.z);
+    return __float_as_int(a.w);
+}
+
+__device__ __forceinline__ int get_ray_path(const ws_view& w, int b, int slot) {
+    return __float_as_int(((const RX_GLOBAL float*)w.q_ray[b])[8 * (long long)slot + 3]);
+}
+
+// Hit records: {t, id, u, v} with the kind folded into the id: a BVH
+// triangle id >= 0, primitive k as -(k + 2), a miss as -1.
+__device__ __forceinline__ int code_hit(int kind, int id) {
+    return kind == RX_KIND_BVH ? id : (kind == RX_KIND_PRIM ? -(id + 2) : -1);
+}
+
+__device__ __forceinline__ void put_hit(const ws_view& w, int i, float t, int kind, int id, float u, float v) {
+    ((RX_GLOBAL gv4f*)w.h_rec)[i] = gv4f{t, __int_as_float(code_hit(kind, id)), u, v};
+}
+
+__device__ __forceinline__ void get_hit(const ws_view& w, int i, float& t, int& kind, int& id, float& u, float& v) {
+    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[i];
+    int c = __float_as_int(h.y);
+    t = h.x;
+    u = h.z;
+    v = h.w;
+    kind = c >= 0 ? RX_KIND_BVH : (c == -1 ? RX_KIND_MISS : RX_KIND_PRIM);
+    id = c >= -1 ? c : -(c + 2);
+}
 
 __device__ __forceinline__ int pass_samples(const rx_job& j) {
     int spp = j.params.samples_per_pass;
@@ -370,13 +395,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
         }
         if (active) {
             int slot = w.q_off[jx] + (int)(p - w.path_off[jx]);
-            w.q_path[0][slot] = (int32_t)p;
-            w.q_ox[0][slot] = org.x;
-            w.q_oy[0][slot] = org.y;
-            w.q_oz[0][slot] = org.z;
-            w.q_dx[0][slot] = rd.x;
-            w.q_dy[0][slot] = rd.y;
-            w.q_dz[0][slot] = rd.z;
+            put_ray(w, 0, slot, (int)p, org, rd);
         }
     }
 }
@@ -401,8 +420,69 @@ __device__ __forceinline__ void closest_prim(const rx_scene& sc, f3 ro, f3 rd, f
     }
 }
 
+// The same scan when every active lane's ray belongs to job j (wave-uniform):
+// the job's primitive records are read with scalar loads, once per wave.
+template <bool COUNT>
+__device__ __forceinline__ void closest_prim_job(const rx_job* jobs, int j, f3 ro, f3 rd, float& prim_t, int& prim_id,
+                                                 trav_count& cnt) {
+    const RX_CONST rx_job* J = (const RX_CONST rx_job*)(jobs + j);
+    const int np = J->scene.n_prims;
+    const rx_prim* prims = J->scene.prims;
+    prim_t = 1e9f;
+    prim_id = -1;
+    for (int k = 0; k < np; ++k) {
+        float t = prim_hit(cload(prims + k), ro, rd);
+        if (COUNT) cnt.prims++;
+        if (t > 0.001f && t < prim_t) {
+            prim_t = t;
+            prim_id = k;
+        }
+    }
+}
+
+// closest_prim for the rays of the active lanes, with scalar loads when they
+// all belong to one job.
+template <bool COUNT>
+__device__ __forceinline__ void closest_prim_lanes(const ws_view& w, int jx, f3 ro, f3 rd, float& prim_t,
+                                                   int& prim_id, trav_count& cnt) {
+    int j0 = __builtin_amdgcn_readfirstlane(jx);
+    if (__ballot(jx != j0) == 0ull)
+        closest_prim_job<COUNT>(w.jobs, j0, ro, rd, prim_t, prim_id, cnt);
+    else
+        closest_prim<COUNT>(w.jobs[jx].scene, ro, rd, prim_t, prim_id, cnt);
+}
+
+// Any primitive occluding (0.001 < t < max_t) for the active lanes' shadow
+// rays, in index order with the first occluder ending the scan
+// (intersect.py:226-241); scalar loads when the lanes share a job.
+template <bool COUNT>
+__device__ __forceinline__ bool any_prim_lanes(const ws_view& w, int jx, f3 so, f3 ld, float max_t,
+                                               trav_count& cnt) {
+    bool occ = false;
+    int j0 = __builtin_amdgcn_readfirstlane(jx);
+    if (__ballot(jx != j0) == 0ull) {
+        const RX_CONST rx_job* J = (const RX_CONST rx_job*)(w.jobs + j0);
+        const int np = J->scene.n_prims;
+        const rx_prim* prims = J->scene.prims;
+        for (int k = 0; k < np && !occ; ++k) {
+            float t = prim_hit(cload(prims + k), so, ld);
+            if (COUNT) cnt.prims++;
+            if (0.001f < t && t < max_t) occ = true;
+        }
+    } else {
+        const rx_scene& sc = w.jobs[jx].scene;
+        const int np = sc.n_prims;
+        for (int k = 0; k < np && !occ; ++k) {
+            float t = prim_hit(gload(sc.prims + k), so, ld);
+            if (COUNT) cnt.prims++;
+            if (0.001f < t && t < max_t) occ = true;
+        }
+    }
+    return occ;
+}
+
 // Closest-primitive pre-pass over ray queue `qi`: writes each ray's
-// primitive hit into h_t / h_id, where k_extend<.., PRE = true> picks it
+// primitive hit into the hit records, where k_extend<.., PRE = true> picks it
 // up.  The BVH walk never reads the primitive result (it starts from
 // closest = 1e9 and is compared with it only at the end, intersect.py:292),
 // so computing it first and separately leaves every hit unchanged.
@@ -414,15 +494,13 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
     for (int base = xcd_block() * BS; base < n; base += stride) {
         int i = base + threadIdx.x;
         if (i < n) {
-            int p = w.q_path[qi][i];
-            const rx_scene& sc = w.jobs[job_of_path(w, p)].scene;
-            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
-            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+            f3 ro, rd;
+            int p = get_ray(w, qi, i, ro, rd);
             float t;
             int id;
-            closest_prim<COUNT>(sc, ro, rd, t, id, cnt);
-            w.h_t[i] = t;
-            w.h_id[i] = id;
+            closest_prim_lanes<COUNT>(w, job_of_path(w, p), ro, rd, t, id, cnt);
+            // the primitive result alone: {prim_t, prim_id}
+            ((RX_GLOBAL gv4f*)w.h_rec)[i] = gv4f{t, __int_as_float(id), 0.0f, 0.0f};
         }
     }
     if (COUNT) wave_add(&ctr_slot(w)->prim_tests, cnt.prims);
@@ -471,18 +549,18 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
         {
             if (idle && mine < n) {
                 i = mine;
-                p = w.q_path[qi][i];
-                sc = &w.jobs[job_of_path(w, p)].scene;
-                ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
-                rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
+                p = get_ray(w, qi, i, ro, rd);
+                const int jx = job_of_path(w, p);
+                sc = &w.jobs[jx].scene;
                 rays++;
                 // primitive scan first (intersect.py:259-287), or its
                 // result from k_prim_scan
                 if (PRE) {
-                    prim_t = w.h_t[i];
-                    prim_id = w.h_id[i];
+                    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[i];
+                    prim_t = h.x;
+                    prim_id = __float_as_int(h.y);
                 } else {
-                    closest_prim<COUNT>(*sc, ro, rd, prim_t, prim_id, cnt);
+                    closest_prim_lanes<COUNT>(w, jx, ro, rd, prim_t, prim_id, cnt);
                 }
                 best_t = 1e9f;
                 best_i = -1;
@@ -550,11 +628,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     u = best_u;
                     v = best_v;
                 }
-                w.h_t[i] = t;
-                w.h_id[i] = id;
-                w.h_kind[i] = kind;
-                w.h_u[i] = u;
-                w.h_v[i] = v;
+                put_hit(w, i, t, kind, id, u, v);
                 w.p_rays[p] += 1;
                 i = -1;
             }
@@ -640,7 +714,8 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
         f3 hit_p = mk(0.0f, 0.0f, 0.0f), nrm = hit_p, alb = hit_p, thr0 = hit_p, ro2 = hit_p, rd2 = hit_p;
         float scale = 1.0f;
         if (valid) {
-            p = w.q_path[qi][i];
+            f3 ro, rd;
+            p = get_ray(w, qi, i, ro, rd);
             int jx = job_of_path(w, p);
             const rx_job& J = w.jobs[jx];
             const rx_scene& sc = J.scene;
@@ -650,10 +725,9 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
             int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
             bool last = depth + 1 >= prm.max_depth;
             bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
-            f3 ro = mk(w.q_ox[qi][i], w.q_oy[qi][i], w.q_oz[qi][i]);
-            f3 rd = mk(w.q_dx[qi][i], w.q_dy[qi][i], w.q_dz[qi][i]);
-            float t = w.h_t[i];
-            int kind = w.h_kind[i], id = w.h_id[i];
+            float t, hu, hv;
+            int kind, id;
+            get_hit(w, i, t, kind, id, hu, hv);
             f3 thr = mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
             f3 col = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
             if (t < 0.0f || t >= 1e9f) {
@@ -671,7 +745,7 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
                 f3 nn;
                 if (kind == RX_KIND_BVH) {
                     const rx_tri_shade sh = gload(sc.tri_shade + id);
-                    float u = w.h_u[i], v = w.h_v[i];
+                    float u = hu, v = hv;
                     float ww = (1.0f - u) - v;
                     nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
                     m = gload(sc.materials + sh.mat);
@@ -763,13 +837,7 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
         }
         int slot = block_append(&w.counts[qo], cont, app);
         if (cont) {
-            w.q_path[qo][slot] = p;
-            w.q_ox[qo][slot] = ro2.x;
-            w.q_oy[qo][slot] = ro2.y;
-            w.q_oz[qo][slot] = ro2.z;
-            w.q_dx[qo][slot] = rd2.x;
-            w.q_dy[qo][slot] = rd2.y;
-            w.q_dz[qo][slot] = rd2.z;
+            put_ray(w, qo, slot, p, ro2, rd2);
         }
         int es = block_append(&w.counts[2], want_nee, app);
         if (want_nee) {
@@ -843,7 +911,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     bool exhausted = false;
     int i = -1;  // request of this lane, -1 when idle
     // request state
-    int p = 0, pix = 0, frame = 0, L = 0, li = 0, l_end = 0, cast = 0;
+    int p = 0, pix = 0, frame = 0, L = 0, li = 0, l_end = 0, cast = 0, jcur = 0;
     bool all = false;
     const rx_job* J = nullptr;
     f3 hp = mk(0.0f, 0.0f, 0.0f), nn = hp, alb = hp, direct = hp;
@@ -861,6 +929,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                 i = mine;
                 p = w.e_path[i];
                 int jx = job_of_path(w, p);
+                jcur = jx;
                 J = &w.jobs[jx];
                 int npix = J->camera.width * J->camera.height;
                 int local = p - w.path_off[jx];
@@ -904,14 +973,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                     cast++;
                     tracing = true;
                     // primitive scan, then the BVH (intersect.py:222-246)
-                    occ = false;
-                    const int np = J->scene.n_prims;
-                    const rx_prim* prims = J->scene.prims;
-                    for (int k2 = 0; k2 < np && !occ; ++k2) {
-                        float t = prim_hit(gload(prims + k2), so, ld);
-                        if (COUNT) cnt.prims++;
-                        if (0.001f < t && t < max_t) occ = true;
-                    }
+                    occ = any_prim_lanes<COUNT>(w, jcur, so, ld, max_t, cnt);
                     have = false;
                     if (!occ && J->scene.n_tris > 0) {
                         const rx_bvh_header hdr = gload(J->scene.header);
