@@ -21,6 +21,8 @@
  *                    rendering/taichi_renderer.py:140-156 (one launch per sample)
  *   rx_denoise       rendering/denoise.py:23-104 (edge_aware_denoise)
  *   rx_display       rendering/denoise.py:107-108 (linear_to_display)
+ *   rx_flatten_indexed  scene/mesh.py:397-430 (indexed_triangle_arrays) via
+ *                    rendering/taichi/extractor.py:174-190
  */
 #ifndef ROXY_HIP_H
 #define ROXY_HIP_H
@@ -360,6 +362,22 @@ int rx_denoise(const float* d_color, const float* d_normal, const float* d_albed
 
 /* sqrt(min(max(x, 0), 1)) elementwise over n floats. */
 int rx_display(const float* d_linear, int n, float* d_out, void* stream);
+
+/* ---- scene flattening: scene/mesh.py:397-430 (indexed_triangle_arrays,
+ * with the transforms of :547-568) as rendering/taichi/extractor.py:174-190
+ * calls it for an IndexedMesh object.  HOST pointers: the caller's numpy
+ * arrays.  Corner positions through `matrix` (row-major 4x4, perspective
+ * divide where |w| > 1e-12), per-vertex normals (or the face normal of the
+ * untransformed corners where a triangle has no normal indices) through
+ * the 3x3 part of `normal_matrix`, then normalised; uvs where all three
+ * indices are >= 0, else zeros.  fp64 in a fixed order, fp32 out; a NULL
+ * matrix is the identity.  threads <= 0 uses every host core. */
+int rx_flatten_indexed(const double* positions, int n_positions, const int32_t* tri_pos_idx, int n_tris,
+                       const double* normals, int n_normals, const int32_t* tri_normal_idx,
+                       const double* uvs, int n_uvs, const int32_t* tri_uv_idx,
+                       const double* matrix, const double* normal_matrix,
+                       float* v0, float* v1, float* v2, float* n0, float* n1, float* n2,
+                       float* uv0, float* uv1, float* uv2, int32_t* has_uv, int threads);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,8 @@ SIGNATURES = {
     "rx_denoise": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                              C.c_float, C.c_float, vp, vp]),
     "rx_display": (C.c_int, [vp, C.c_int, vp, vp]),
+    "rx_flatten_indexed": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp,
+                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]),
 }
 
 _LIB = None

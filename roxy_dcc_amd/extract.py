@@ -2,7 +2,8 @@
 
 Walks the object tree exactly as ``_collect`` does (:134-205): a ``Mesh``
 (per-triangle objects) is transformed triangle by triangle, an
-``IndexedMesh`` through its vectorised ``indexed_triangle_arrays``, and any
+``IndexedMesh`` by the library's native flattener (``rx_flatten_indexed``,
+the fp64 transform of ``indexed_triangle_arrays`` on every host core), and any
 other geometry is exported once per object with ``taichi_export()``.
 Lights are the exported spheres with material type 3 and emission > 0
 (:36-40).  Per-triangle ``Mesh`` data is appended as one batch after the
@@ -14,8 +15,11 @@ indexed batches (:223-224), as in the reference.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
+from . import _native as nat
 from .bvh import TriangleBatch
 from .scene import DeviceScene, FlatScene, PrimitiveSlots
 
@@ -56,6 +60,44 @@ def _is_object_mesh(geo):
     return hasattr(geo, "_triangles")
 
 
+def _host_threads():
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return n if n > 0 else min(16, os.cpu_count() or 1)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def indexed_arrays_native(geo, matrix=None, normal_matrix=None):
+    """``IndexedMesh.indexed_triangle_arrays`` (scene/mesh.py:397-430) by
+    the library's native flattener: same keys, fp32 arrays."""
+    n = geo.triangle_count
+    out = {k: np.empty((n, 3), np.float32) for k in ("v0", "v1", "v2", "n0", "n1", "n2")}
+    out.update({k: np.empty((n, 2), np.float32) for k in ("uv0", "uv1", "uv2")})
+    has_uv = np.empty(n, np.int32)
+    pos = np.ascontiguousarray(geo._positions, dtype=np.float64)
+    tpi = np.ascontiguousarray(geo._tri_pos_idx, dtype=np.int32)
+    nrm = None if geo._normals is None else np.ascontiguousarray(geo._normals, dtype=np.float64)
+    tni = np.ascontiguousarray(geo._tri_normal_idx, dtype=np.int32)
+    uvs = None if geo._uvs is None else np.ascontiguousarray(geo._uvs, dtype=np.float64)
+    tui = np.ascontiguousarray(geo._tri_uv_idx, dtype=np.int32)
+    m = None if matrix is None else np.ascontiguousarray(matrix.rows, dtype=np.float64)
+    nm = None if normal_matrix is None else np.ascontiguousarray(normal_matrix.rows, dtype=np.float64)
+    nat.check(nat.lib().rx_flatten_indexed(
+        _ptr(pos), len(pos), _ptr(tpi), n, _ptr(nrm), 0 if nrm is None else len(nrm), _ptr(tni),
+        _ptr(uvs), 0 if uvs is None else len(uvs), _ptr(tui), _ptr(m), _ptr(nm),
+        *[_ptr(out[k]) for k in ("v0", "v1", "v2", "n0", "n1", "n2", "uv0", "uv1", "uv2")], _ptr(has_uv),
+        _host_threads()), "flatten")
+    out["has_uv"] = has_uv
+    out["group_idx"] = np.ascontiguousarray(geo._tri_group_idx, dtype=np.int32)
+    out["groups"] = list(geo._groups)
+    return out
+
+
 def _uv2(uv):
     return [float(uv[0]), float(uv[1])] if uv is not None else [0.0, 0.0]
 
@@ -93,7 +135,7 @@ def flatten_scene(world) -> FlatScene:
                         **_material_data(mat),
                     })
             elif _is_indexed(geo):
-                arrays = geo.indexed_triangle_arrays(obj.world_matrix, obj.world_inverse_transpose_matrix)
+                arrays = indexed_arrays_native(geo, obj.world_matrix, obj.world_inverse_transpose_matrix)
                 by_group = {}
                 for gi, group in enumerate(arrays["groups"]):
                     mat = shape.material_for_group(group) or _default_material()

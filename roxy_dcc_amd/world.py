@@ -215,28 +215,36 @@ def _rows(matrix):
 
 
 def _xf_points(pts, matrix):
+    """scene/mesh.py:547-558 with the products spelled out elementwise in
+    fp64: ((x m_j0 + y m_j1) + z m_j2) + m_j3, perspective divide where
+    |w| > 1e-12.  (The reference writes a numpy matmul, whose rounding is the
+    linked BLAS's; the native flattener uses exactly this order.)"""
     m = _rows(matrix)
     out = np.asarray(pts, dtype=np.float64)
     if m is None:
         return out.copy()
     flat = out.reshape(-1, 3)
-    res = flat @ m[:3, :3].T + m[:3, 3]
-    w = flat @ m[3, :3].T + m[3, 3]
+    x, y, z = flat[:, 0], flat[:, 1], flat[:, 2]
+    res = np.stack([((x * m[j, 0] + y * m[j, 1]) + z * m[j, 2]) + m[j, 3] for j in range(3)], axis=1)
+    w = ((x * m[3, 0] + y * m[3, 1]) + z * m[3, 2]) + m[3, 3]
     div = np.abs(w) > 1e-12
     res[div] /= w[div, None]
     return res.reshape(out.shape)
 
 
 def _xf_vectors(vecs, matrix):
+    """scene/mesh.py:560-567, elementwise: (x m_j0 + y m_j1) + z m_j2."""
     m = _rows(matrix)
     out = np.asarray(vecs, dtype=np.float64)
     if m is None:
         return out.copy()
-    return (out.reshape(-1, 3) @ m[:3, :3].T).reshape(out.shape)
+    flat = out.reshape(-1, 3)
+    x, y, z = flat[:, 0], flat[:, 1], flat[:, 2]
+    return np.stack([(x * m[j, 0] + y * m[j, 1]) + z * m[j, 2] for j in range(3)], axis=1).reshape(out.shape)
 
 
 def _normalize_rows(rows):
-    lengths = np.linalg.norm(rows, axis=1)
+    lengths = np.sqrt((rows[:, 0] * rows[:, 0] + rows[:, 1] * rows[:, 1]) + rows[:, 2] * rows[:, 2])
     out = rows.copy()
     ok = lengths > 1e-12
     out[ok] /= lengths[ok, None]

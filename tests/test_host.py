@@ -335,3 +335,82 @@ def test_share_pass_water_filling():
     assert all(1 <= s <= need for s, (_, need) in zip(fifo, demands))
     assert fifo[0] == 4 and fifo[1] == 23 and fifo[2] == 1 and fifo[3] == 16
     assert sum(s * px for s, (px, _) in zip(fifo, demands)) <= 1 << 23
+
+
+def _random_indexed_mesh(rng, n_pos=200, n_tris=500, normals=True, uvs=True):
+    from roxy_dcc_amd.world import IndexedMesh
+
+    pos = rng.normal(size=(n_pos, 3)) * 3.0
+    tri = rng.integers(0, n_pos, size=(n_tris, 3)).astype(np.int32)
+    tri[:5, 1] = tri[:5, 0]                       # degenerate: zero-area faces
+    kw = {}
+    if normals:
+        nrm = rng.normal(size=(n_pos, 3))
+        nrm[:3] = 0.0                             # zero normals stay unnormalised
+        tni = rng.integers(0, n_pos, size=(n_tris, 3)).astype(np.int32)
+        tni[::7, 2] = -1                          # these triangles fall back to the face normal
+        kw.update(normals=nrm, tri_normal_idx=tni)
+    if uvs:
+        kw.update(uvs=rng.random((50, 2)), tri_uv_idx=np.where(rng.random((n_tris, 3)) < 0.9,
+                                                               rng.integers(0, 50, (n_tris, 3)), -1).astype(np.int32))
+    return IndexedMesh(pos, tri, **kw)
+
+
+def _random_matrices(rng):
+    from roxy_dcc_amd.world import Mat4
+
+    m = Mat4.trs(translation=rng.normal(size=3) * 10, rotation_deg=rng.uniform(-180, 180, 3),
+                 scale=rng.uniform(0.2, 3.0, 3))
+    p = Mat4(m.rows.copy())
+    p.rows[3] = [0.01, -0.02, 0.03, 1.0]          # a projective row exercises the divide
+    return [(None, None), (m, m.inverse().transpose()), (p, p.inverse().transpose())]
+
+
+def test_native_flatten_equals_restatement():
+    """rx_flatten_indexed equals the fp64 numpy restatement of
+    indexed_triangle_arrays bit for bit: affine and projective matrices,
+    identity, per-vertex and face normals, missing uvs, degenerate faces."""
+    from roxy_dcc_amd.extract import indexed_arrays_native
+
+    rng = np.random.default_rng(17)
+    for normals, uvs in [(True, True), (False, False), (True, False)]:
+        mesh = _random_indexed_mesh(rng, normals=normals, uvs=uvs)
+        for m, nm in _random_matrices(rng):
+            a = mesh.indexed_triangle_arrays(m, nm)
+            b = indexed_arrays_native(mesh, m, nm)
+            for k in ("v0", "v1", "v2", "n0", "n1", "n2", "uv0", "uv1", "uv2", "has_uv", "group_idx"):
+                assert a[k].dtype == b[k].dtype and np.array_equal(a[k], b[k]), (k, normals, uvs)
+
+
+def test_flatten_restatement_matches_reference_matmul_form():
+    """The elementwise fp64 order agrees with the reference's matmul form
+    (scene/mesh.py:547-568: flat @ m[:3,:3].T + m[:3,3]) to within one fp32
+    ulp, and exactly for almost every coordinate."""
+    from roxy_dcc_amd.world import _xf_points, _xf_vectors
+
+    rng = np.random.default_rng(5)
+    pts = rng.normal(size=(20000, 3)) * 50
+    for m, nm in _random_matrices(rng)[1:]:
+        r = m.rows
+        ref = pts @ r[:3, :3].T + r[:3, 3]
+        w = pts @ r[3, :3].T + r[3, 3]
+        div = np.abs(w) > 1e-12
+        ref[div] /= w[div, None]
+        got = _xf_points(pts, m)
+        a, b = ref.astype(np.float32), got.astype(np.float32)
+        assert np.all(np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64)) <= 1)
+        assert np.mean(a == b) > 0.999
+        vref = (pts @ nm.rows[:3, :3].T).astype(np.float32)
+        vgot = _xf_vectors(pts, nm).astype(np.float32)
+        assert np.all(np.abs(vref.view(np.int32).astype(np.int64) - vgot.view(np.int32).astype(np.int64)) <= 1)
+
+
+def test_native_flatten_rejects_bad_indices():
+    import pytest
+
+    from roxy_dcc_amd.extract import indexed_arrays_native
+    from roxy_dcc_amd.world import IndexedMesh
+
+    mesh = IndexedMesh(np.zeros((3, 3)), np.array([[0, 1, 3]], np.int32))
+    with pytest.raises(ValueError):
+        indexed_arrays_native(mesh)
