@@ -303,12 +303,16 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
     for i in range(n_requests):
         key = keys[int(rng.integers(len(keys)))]
         W, H = sizes[int(rng.choice(len(sizes), p=[0.3, 0.3, 0.2, 0.15, 0.05]))]
-        spp = int(rng.choice([4, 8, 16, 32, 64]))
-        spec.append((key, W, H, spp))
+        # a quarter of the requests carry an error target (the adaptive stop
+        # of rendering/ray_tracer.py:531-541, threshold 0.005, at most 64 spp)
+        if rng.random() < 0.25:
+            spec.append((key, W, H, 64, (0.005, 4, 1)))
+        else:
+            spec.append((key, W, H, int(rng.choice([4, 8, 16, 32, 64])), None))
 
     def requests(arrivals):
-        return [Request(i, key, W, H, spp, float(a), max_depth=6, seed=1000 + i)
-                for i, ((key, W, H, spp), a) in enumerate(zip(spec, arrivals))]
+        return [Request(i, key, W, H, spp, float(a), max_depth=6, seed=1000 + i, adaptive=ad)
+                for i, ((key, W, H, spp, ad), a) in enumerate(zip(spec, arrivals))]
 
     cache = SceneCache(load, max_scenes=32)
     t0 = time.perf_counter()
@@ -321,6 +325,7 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
     ref = serial_loop(cache, requests([0.0] * n_requests))
     cap = n_requests / (time.perf_counter() - t0)
     ref_frames = {r.rid: r.frame for r in ref}
+    adaptive_spp = [r.samples_done for r in ref if spec[r.rid][4] is not None]
     rows = []
     for f in loads:
         rate = f * cap
@@ -342,11 +347,13 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
         row["identical_images"] = same
         rows.append(row)
     return {"workload": f"{n_requests} requests over {len(keys)} resident scenes (6 mesh, 2 large), "
-                        "128^2..1920x1080, 4..64 spp, depth 6, Poisson arrivals; offered load relative to the "
+                        "128^2..1920x1080, 4..64 spp or an error target (adaptive stop, threshold 0.005), depth 6, "
+                        "Poisson arrivals; offered load relative to the "
                         "serial loop's capacity; serial = passes of up to 4M paths, serial_per_sample = one "
                         "sample per pass (the reference's launch-per-sample model); identical_images counts "
                         "both server policies against the serial images",
-            "scene_setup_s": setup, "serial_capacity_per_s": cap, "sweep": rows}
+            "scene_setup_s": setup, "serial_capacity_per_s": cap, "error_target_requests": len(adaptive_spp),
+            "error_target_samples_taken": adaptive_spp, "sweep": rows}
 
 
 def main():

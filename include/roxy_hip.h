@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define RX_ABI_VERSION 1
+#define RX_ABI_VERSION 2
 
 /* status codes */
 #define RX_OK 0
@@ -147,9 +147,23 @@ typedef struct rx_light {
     float emission;
 } rx_light;
 
-/* Scene view: device pointers plus host-side sizes.  Every array must be
- * 16-byte aligned (records are read as 16-byte words); the entry points
- * return RX_ERR_INVALID otherwise. */
+/* Image texture of a scene's texture cache (fields.py:82-88): u8 RGB
+ * texels, row-major rows of 3*width bytes, starting `offset` bytes into
+ * the scene's texel pool; flip_v as ImageTexture.flip_v.  32 bytes. */
+typedef struct rx_texture {
+    int32_t width;
+    int32_t height;
+    int32_t flip_v;
+    int32_t pad0;
+    long long offset;
+    long long pad1;
+} rx_texture;
+
+/* Scene view: device pointers plus host-side sizes.  Every record array must
+ * be 16-byte aligned (records are read as 16-byte words); the entry points
+ * return RX_ERR_INVALID otherwise.  tri_uv (6 floats per triangle in leaf
+ * order: uv0, uv1, uv2), textures and texels are needed only when a
+ * material has a texture (material.texture >= 0, trace.py:129-136). */
 typedef struct rx_scene {
     const rx_node* nodes;
     const rx_bvh_header* header;
@@ -159,12 +173,17 @@ typedef struct rx_scene {
     const rx_prim* prims;
     const rx_material* prim_materials;
     const rx_light* lights;
+    const float* tri_uv;
+    const rx_texture* textures;
+    const unsigned char* texels;
     int32_t n_nodes;
     int32_t n_tris;
     int32_t n_materials;
     int32_t n_prims;
     int32_t n_lights;
     int32_t bvh_depth;
+    int32_t n_textures;
+    int32_t pad;
 } rx_scene;
 
 /* Pinhole camera (rendering/taichi/camera.py:23-30). */
@@ -184,6 +203,8 @@ typedef struct rx_camera {
 #define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
 #define RX_FLAG_PRIM_PREPASS 4   /* closest-primitive scan as its own pass over each ray
                                     queue (automatic above 256 analytic primitives) */
+#define RX_FLAG_ADAPTIVE 16      /* adaptive stop: rendering/ray_tracer.py:531-541 (see
+                                    adaptive_* below) */
 #define RX_FLAG_SMALL_BLOCKS 8   /* traversal in 64-thread blocks with the deep stack (the
                                     shape used anyway for BVHs deeper than 23).  The batch
                                     workspace query sees it; for rx_render size the
@@ -204,6 +225,15 @@ typedef struct rx_render_params {
     int32_t trav_blocks_per_cu;      /* resident blocks per CU of the persistent traversal
                                         kernels: 0 = the register budget's maximum (8 blocks
                                         of 256, or 32 of 64 threads), else 1 .. that maximum */
+    /* With RX_FLAG_ADAPTIVE (rendering/ray_tracer.py:531-541): after sample n
+     * (n = samples rendered, n >= 2) the error is the mean over pixels and
+     * channels of |mean after n - mean after n-1| (linear colour); the job
+     * stops after sample n when n < its requested total, n >= min_samples,
+     * (n - min_samples) % check_interval == 0 and error <= threshold.  A pass
+     * whose samples run past the stop accumulates only up to it. */
+    float adaptive_threshold;
+    int32_t adaptive_min_samples;
+    int32_t adaptive_check_interval;
 } rx_render_params;
 
 /* Per-pixel running means written by rx_render (fields.py:14-19). */
@@ -289,6 +319,10 @@ int rx_bvh_pack_triangles(const int32_t* d_order, int n_tris,
                           const float* d_n0, const float* d_n1, const float* d_n2,
                           const int32_t* d_mat_idx, const int32_t* d_has_uv,
                           rx_tri* d_tris, rx_tri_shade* d_shade, void* stream);
+/* Gather per-corner UVs (n x 2 each) into leaf order, 6 floats per
+ * triangle (the _bvh_uv* fields of fields.py:65-67 in BVH order). */
+int rx_bvh_pack_uvs(const int32_t* d_order, int n_tris, const float* d_uv0, const float* d_uv1,
+                    const float* d_uv2, float* d_tri_uv, void* stream);
 
 /* ---- queries: rendering/taichi/intersect.py ------------------------------
  * d_rays: 6*n floats, SoA (ox[n], oy[n], oz[n], dx[n], dy[n], dz[n]).
@@ -334,6 +368,8 @@ typedef struct rx_job {
     int32_t path_offset;       /* set by the library: first path slot */
     int32_t pixel_offset;      /* set by the library: first pixel slot */
     unsigned long long rays;   /* rays cast for this job (accumulated) */
+    float adaptive_error;      /* last adaptive error (RX_FLAG_ADAPTIVE), -1 before one exists */
+    int32_t adaptive_stopped;  /* 1 once the adaptive rule stopped the job */
 } rx_job;
 
 typedef struct rx_batch rx_batch;

@@ -32,7 +32,9 @@ class OrcScene(C.Structure):
                 ("obj_extra", fp), ("obj_v0", fp), ("obj_v1", fp), ("obj_v2", fp), ("obj_mat_type", ip),
                 ("obj_albedo", fp), ("obj_roughness", fp), ("obj_ior", fp), ("obj_emission", fp),
                 ("n_lights", C.c_int), ("light_center", fp), ("light_radius", fp), ("light_albedo", fp),
-                ("light_emission", fp)]
+                ("light_emission", fp), ("mat_texture", ip), ("has_uv", ip), ("uv0", fp), ("uv1", fp), ("uv2", fp),
+                ("n_textures", C.c_int), ("tex_width", ip), ("tex_height", ip), ("tex_flip_v", ip),
+                ("tex_offset", C.POINTER(C.c_longlong)), ("texels", C.POINTER(C.c_ubyte))]
 
 
 class OrcCamera(C.Structure):
@@ -89,7 +91,11 @@ def threads_default():
 class OracleScene:
     """Scene in the reference field layout, BVH from bvh_oracle."""
 
-    def __init__(self, v0, v1, v2, n0, n1, n2, tri_mat, palette, prims=None, builder="morton", max_leaf=8):
+    def __init__(self, v0, v1, v2, n0, n1, n2, tri_mat, palette, prims=None, builder="morton", max_leaf=8,
+                 uvs=None, has_uv=None, textures=None):
+        """uvs: (uv0, uv1, uv2) per triangle (n x 2 each); has_uv per triangle;
+        textures: list of (u8 HxWx3 texels, flip_v) indexed by the palette's
+        texture column."""
         self.keep = []
         tri_mat = np.asarray(tri_mat, np.int32)
         n = len(tri_mat)
@@ -114,6 +120,22 @@ class OracleScene:
         self.mat_roughness = np.asarray([p[2] for p in pal], np.float32)
         self.mat_ior = np.asarray([p[3] for p in pal], np.float32)
         self.mat_emission = np.asarray([p[4] for p in pal], np.float32)
+        self.mat_texture = np.asarray([p[5] if len(p) > 5 else -1 for p in pal], np.int32)
+        if uvs is not None and n:
+            self.uv = [np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 2)[o]) for u in uvs]
+        else:
+            self.uv = [np.zeros((max(n, 1), 2), np.float32) for _ in range(3)]
+        self.has_uv = (np.ascontiguousarray(np.asarray(has_uv, np.int32)[o]) if has_uv is not None and n
+                       else np.zeros(max(n, 1), np.int32))
+        tex = list(textures or [])
+        self.tex_width = np.asarray([t[0].shape[1] for t in tex] or [0], np.int32)
+        self.tex_height = np.asarray([t[0].shape[0] for t in tex] or [0], np.int32)
+        self.tex_flip = np.asarray([1 if t[1] else 0 for t in tex] or [0], np.int32)
+        sizes = [t[0].size for t in tex]
+        self.tex_offset = np.asarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]) if tex else [0], np.int64)
+        self.texels = (np.ascontiguousarray(np.concatenate([np.asarray(t[0], np.uint8).reshape(-1) for t in tex]))
+                       if tex else np.zeros(1, np.uint8))
+        self.n_textures = len(tex)
         t = self.tree
         if t.n_nodes == 0:
             t = bvh_oracle.Tree(1)
@@ -164,6 +186,13 @@ class OracleScene:
         s.n_lights = nl
         for name, arr in self.lights.items():
             setattr(s, name, _f(arr))
+        s.mat_texture = _i(self.mat_texture)
+        s.has_uv = _i(self.has_uv)
+        s.uv0, s.uv1, s.uv2 = (_f(u) for u in self.uv)
+        s.n_textures = self.n_textures
+        s.tex_width, s.tex_height, s.tex_flip_v = _i(self.tex_width), _i(self.tex_height), _i(self.tex_flip)
+        s.tex_offset = self.tex_offset.ctypes.data_as(C.POINTER(C.c_longlong))
+        s.texels = self.texels.ctypes.data_as(C.POINTER(C.c_ubyte))
 
 
 def closest_hit(sc: OracleScene, rays, threads=None):
@@ -239,6 +268,33 @@ def render(sc: OracleScene, cam: OrcCamera, prm: OrcParams, first, count, acc: A
        _i(plist) if plist is not None else None, nl)
     acc.samples = first + count
     return acc
+
+
+def adaptive_errors(sc: OracleScene, cam: OrcCamera, prm: OrcParams, samples, acc: Accum, threads=None):
+    """rendering/ray_tracer.py:497-541 without the stop: renders one sample
+    at a time and returns e[n] (n = samples rendered, from 2) = mean over
+    pixels and channels of |accum_n - accum_(n-1)|, the error the adaptive
+    rule compares with its threshold (:534), in float64 of the accumulator."""
+    errs = {}
+    prev = None
+    for frame in range(samples):
+        render(sc, cam, prm, frame, 1, acc, threads)
+        cur = np.asarray(acc.color, dtype=np.float64).copy()
+        if prev is not None:
+            errs[frame + 1] = float(np.mean(np.abs(cur - prev)))
+        prev = cur
+    return errs
+
+
+def adaptive_stop(errs, samples, min_samples, threshold, interval):
+    """The sample count at which rendering/ray_tracer.py:531-541 stops (or
+    `samples`), given the errors e[n] of adaptive_errors."""
+    for n in range(2, samples + 1):
+        if n >= samples or n < min_samples or (n - min_samples) % interval:
+            continue
+        if errs[n] <= threshold:
+            return n
+    return samples
 
 
 def philox(ctr, key):

@@ -82,6 +82,19 @@ typedef struct orc_scene {
     const float* light_radius;
     const float* light_albedo;
     const float* light_emission;
+    // textures (fields.py:65-67 uvs in BVH order, :78 material texture,
+    // :82-88 cache); mat_texture NULL = no textures
+    const int32_t* mat_texture;
+    const int32_t* has_uv;
+    const float* uv0;
+    const float* uv1;
+    const float* uv2;
+    int n_textures;
+    const int32_t* tex_width;
+    const int32_t* tex_height;
+    const int32_t* tex_flip_v;
+    const long long* tex_offset;
+    const unsigned char* texels;
 } orc_scene;
 
 typedef struct orc_camera {
@@ -634,6 +647,28 @@ template <class R> struct path_out {
 };
 
 // rendering/taichi/trace.py:139-236
+// rendering/taichi/trace.py:96-126: repeat, optional v flip, bilinear of
+// u8 texels / 255; a zero-size slot samples as white.
+template <class R>
+V<R> sample_texture(const orc_scene& s, int t, R tu, R tv) {
+    int W = s.tex_width[t], H = s.tex_height[t];
+    if (W <= 0 || H <= 0) return v3<R>(1, 1, 1);
+    R u = tu - std::floor(tu);
+    R v = tv - std::floor(tv);
+    if (s.tex_flip_v[t] != 0) v = (R)1 - v;
+    R x = u * (R)(W - 1), y = v * (R)(H - 1);
+    int x0 = (int)std::floor(x), y0 = (int)std::floor(y);
+    int x1 = std::min(x0 + 1, W - 1), y1 = std::min(y0 + 1, H - 1);
+    R fx = x - (R)x0, fy = y - (R)y0;
+    auto texel = [&](int yy, int xx) {
+        const unsigned char* q = s.texels + s.tex_offset[t] + 3 * ((long long)yy * W + xx);
+        return v3<R>((R)q[0] / (R)255, (R)q[1] / (R)255, (R)q[2] / (R)255);
+    };
+    V<R> c0 = add(scl(texel(y0, x0), (R)1 - fx), scl(texel(y0, x1), fx));
+    V<R> c1 = add(scl(texel(y1, x0), (R)1 - fx), scl(texel(y1, x1), fx));
+    return add(scl(c0, (R)1 - fy), scl(c1, fy));
+}
+
 template <class R>
 path_out<R> trace(const orc_scene& s, const orc_params& p, int pix, int frame, V<R> ro, V<R> rd, orc_counts& cnt,
                   tally& tl, tally& tl_shadow) {
@@ -665,6 +700,12 @@ path_out<R> trace(const orc_scene& s, const orc_params& p, int pix, int frame, V
             int m = s.mat_idx[i];
             mtype = s.mat_type[m];
             alb = at<R>(s.mat_albedo, m);
+            // trace.py:129-136 (_bvh_albedo)
+            if (s.mat_texture && s.mat_texture[m] >= 0 && s.has_uv[i] != 0 && s.mat_texture[m] < s.n_textures) {
+                R tu = (s.uv0[2 * i] * w + s.uv1[2 * i] * h.u) + s.uv2[2 * i] * h.v;
+                R tv = (s.uv0[2 * i + 1] * w + s.uv1[2 * i + 1] * h.u) + s.uv2[2 * i + 1] * h.v;
+                alb = hmul(alb, sample_texture<R>(s, s.mat_texture[m], tu, tv));
+            }
             rough = s.mat_roughness[m];
             ior = s.mat_ior[m];
             emis = s.mat_emission[m];

@@ -31,6 +31,7 @@ RX_FLAG_PROFILE = 1
 RX_FLAG_COUNT_TRAFFIC = 2
 RX_FLAG_PRIM_PREPASS = 4
 RX_FLAG_SMALL_BLOCKS = 8
+RX_FLAG_ADAPTIVE = 16
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -47,10 +48,15 @@ i32 = C.c_int32
 class RxScene(C.Structure):
     _fields_ = [
         ("nodes", vp), ("header", vp), ("tris", vp), ("tri_shade", vp), ("materials", vp),
-        ("prims", vp), ("prim_materials", vp), ("lights", vp),
+        ("prims", vp), ("prim_materials", vp), ("lights", vp), ("tri_uv", vp), ("textures", vp), ("texels", vp),
         ("n_nodes", i32), ("n_tris", i32), ("n_materials", i32), ("n_prims", i32), ("n_lights", i32),
-        ("bvh_depth", i32),
+        ("bvh_depth", i32), ("n_textures", i32), ("pad", i32),
     ]
+
+
+class RxTexture(C.Structure):
+    _fields_ = [("width", i32), ("height", i32), ("flip_v", i32), ("pad0", i32), ("offset", C.c_longlong),
+                ("pad1", C.c_longlong)]
 
 
 class RxCamera(C.Structure):
@@ -65,6 +71,7 @@ class RxRenderParams(C.Structure):
         ("max_depth", i32), ("use_sky", i32), ("direct_light_mode", i32), ("direct_light_max_depth", i32),
         ("sample_clamp", C.c_float), ("bg", C.c_float * 3), ("seed_lo", C.c_uint32), ("seed_hi", C.c_uint32),
         ("flags", i32), ("samples_per_pass", i32), ("trav_blocks_per_cu", i32),
+        ("adaptive_threshold", C.c_float), ("adaptive_min_samples", i32), ("adaptive_check_interval", i32),
     ]
 
 
@@ -90,7 +97,7 @@ class RxProfile(C.Structure):
 class RxJob(C.Structure):
     _fields_ = [("scene", RxScene), ("camera", RxCamera), ("params", RxRenderParams), ("accum", RxAccum),
                 ("first_sample", i32), ("samples_left", i32), ("path_offset", i32), ("pixel_offset", i32),
-                ("rays", C.c_ulonglong)]
+                ("rays", C.c_ulonglong), ("adaptive_error", C.c_float), ("adaptive_stopped", i32)]
 
 
 class RxBvhRef(C.Structure):
@@ -121,6 +128,7 @@ SIGNATURES = {
     "rx_denoise": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                              C.c_float, C.c_float, vp, vp]),
     "rx_display": (C.c_int, [vp, C.c_int, vp, vp]),
+    "rx_bvh_pack_uvs": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
     "rx_flatten_indexed": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, vp, vp,
                                      vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]),
 }

@@ -34,6 +34,9 @@ class RenderJob:
     samples_per_pass: int = 1
     trav_blocks_per_cu: int = 0     # launch shape (see autotune.Tuning)
     flags: int = 0
+    # adaptive stop (rendering/ray_tracer.py:531-541): None, or
+    # (threshold, min_samples, check_interval)
+    adaptive: tuple = None
     first_sample: int = 0
     frame: DeviceFrame = field(default=None, repr=False)
     tag: object = None
@@ -47,11 +50,13 @@ class RenderJob:
         j.camera = camera_struct(self.camera, self.width, self.height)
         j.params = params_struct(self.max_depth, int(self.use_sky), mode, self.direct_light_max_depth,
                                  self.sample_clamp, self.background, self.seed, self.samples_per_pass,
-                                 flags | self.flags, self.trav_blocks_per_cu)
+                                 flags | self.flags, self.trav_blocks_per_cu, self.adaptive)
         j.accum = self.frame.struct()
         j.first_sample = self.first_sample
         j.samples_left = self.samples - self.first_sample
         j.rays = 0
+        j.adaptive_error = -1.0
+        j.adaptive_stopped = 0
         return j
 
     @property

@@ -441,6 +441,24 @@ class GPUBVHBuilder:
             pal[i] = (m["type"], m["albedo"], m["roughness"], m["ior"], m["emission"], m["texture_idx"])
         d["materials"] = torch.from_numpy(pal.view(np.uint8)).to(dev)
 
+    def pack_uvs(self):
+        """Per-corner UVs in leaf order, 6 floats per triangle (the
+        _bvh_uv0..2 fields of fields.py:65-67), for textured materials."""
+        import torch
+
+        b = self._batch
+        n = b.triangle_count
+        if n == 0:
+            return None
+        dev = self.device
+        uv = [torch.from_numpy(np.ascontiguousarray(getattr(b, k), dtype=np.float32)).to(dev)
+              for k in ("uv0", "uv1", "uv2")]
+        self.d["tri_uv"] = torch.empty(6 * n, dtype=torch.float32, device=dev)
+        nat.check(nat.lib().rx_bvh_pack_uvs(self.d["order"].data_ptr(), n, uv[0].data_ptr(), uv[1].data_ptr(),
+                                            uv[2].data_ptr(), self.d["tri_uv"].data_ptr(), nat.stream_ptr()),
+                  "pack uvs")
+        return self.d["tri_uv"]
+
     # ---------------------------------------------------------------- upload
     def upload(self):
         """The tree is already resident; check the reference's capacity

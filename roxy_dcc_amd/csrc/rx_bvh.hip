@@ -470,6 +470,32 @@ extern "C" int rx_bvh_pack(const rx_bvh_ref* ref, int n_nodes, int depth, rx_nod
     return RX_OK;
 }
 
+// Per-corner UVs in leaf order, 6 floats per triangle.
+__global__ void k_pack_uvs(const int32_t* __restrict__ order, int n, const float* __restrict__ uv0,
+                           const float* __restrict__ uv1, const float* __restrict__ uv2, float* __restrict__ out) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    int s = order[i];
+    float* o = out + 6 * (long long)i;
+    o[0] = uv0[2 * s];
+    o[1] = uv0[2 * s + 1];
+    o[2] = uv1[2 * s];
+    o[3] = uv1[2 * s + 1];
+    o[4] = uv2[2 * s];
+    o[5] = uv2[2 * s + 1];
+}
+
+extern "C" int rx_bvh_pack_uvs(const int32_t* d_order, int n_tris, const float* d_uv0, const float* d_uv1,
+                               const float* d_uv2, float* d_tri_uv, void* stream) {
+    RX_REQUIRE(n_tris >= 0 && n_tris <= RX_MAX_TRIANGLES, RX_ERR_CAPACITY, "BVH has %d triangles, limit is %d",
+               n_tris, RX_MAX_TRIANGLES);
+    if (n_tris == 0) return RX_OK;
+    RX_REQUIRE(d_order && d_uv0 && d_uv1 && d_uv2 && d_tri_uv, RX_ERR_INVALID, "NULL argument");
+    k_pack_uvs<<<(n_tris + BS - 1) / BS, BS, 0, rx_stream(stream)>>>(d_order, n_tris, d_uv0, d_uv1, d_uv2, d_tri_uv);
+    RX_CHECK_LAUNCH();
+    return RX_OK;
+}
+
 extern "C" int rx_bvh_pack_triangles(const int32_t* d_order, int n_tris, const float* d_v0, const float* d_v1,
                                      const float* d_v2, const float* d_n0, const float* d_n1, const float* d_n2,
                                      const int32_t* d_mat_idx, const int32_t* d_has_uv, rx_tri* d_tris,

@@ -46,7 +46,7 @@ static inline size_t rx_align_up(size_t x, size_t a) { return (x + a - 1) / a * 
 // Every scene array is 16-byte aligned (records are read as 16-byte words).
 static inline bool rx_scene_aligned(const rx_scene& sc) {
     const void* ptrs[] = {sc.nodes, sc.header, sc.tris, sc.tri_shade, sc.materials, sc.prims, sc.prim_materials,
-                          sc.lights};
+                          sc.lights, sc.textures};
     for (const void* p : ptrs)
         if (reinterpret_cast<uintptr_t>(p) & 15) return false;
     return true;

@@ -198,6 +198,45 @@ __device__ __forceinline__ f3 camera_dir(const rx_camera& cam, int px, int py, i
     return normalize(d);
 }
 
+// ---------------------------------------------------------------- textures
+// rendering/taichi/trace.py:96-126 (_sample_texture): repeat, optional v
+// flip, bilinear between the four texels around (u (W-1), v (H-1)), u8 /
+// 255; a texture of zero size samples as white.
+__device__ __forceinline__ f3 sample_texture(const rx_texture& tx, const unsigned char* __restrict__ texels, float tu,
+                                             float tv) {
+    if (tx.width <= 0 || tx.height <= 0) return mk(1.0f, 1.0f, 1.0f);
+    float u = tu - floorf(tu);
+    float v = tv - floorf(tv);
+    if (tx.flip_v != 0) v = 1.0f - v;
+    float x = u * (float)(tx.width - 1);
+    float y = v * (float)(tx.height - 1);
+    int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    int x1 = min(x0 + 1, tx.width - 1), y1 = min(y0 + 1, tx.height - 1);
+    float fx = x - (float)x0, fy = y - (float)y0;
+    const unsigned char* base = texels + tx.offset;
+    auto texel = [&](int yy, int xx) {
+        const unsigned char* t = base + 3 * ((long long)yy * tx.width + xx);
+        return mk((float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f);
+    };
+    f3 c00 = texel(y0, x0), c10 = texel(y0, x1), c01 = texel(y1, x0), c11 = texel(y1, x1);
+    f3 c0 = c00 * (1.0f - fx) + c10 * fx;
+    f3 c1 = c01 * (1.0f - fx) + c11 * fx;
+    return c0 * (1.0f - fy) + c1 * fy;
+}
+
+// rendering/taichi/trace.py:129-136 (_bvh_albedo): the material's albedo
+// times its texture at the hit's interpolated UV, for triangles with UVs.
+__device__ __forceinline__ f3 bvh_albedo(const rx_scene& sc, const rx_material& m, int tri, int has_uv, float bu,
+                                         float bv, f3 base) {
+    if (m.texture < 0 || has_uv == 0) return base;
+    if (m.texture >= sc.n_textures) return base;  // an empty cache slot samples as white
+    const float* t = sc.tri_uv + 6 * (long long)tri;
+    float w = (1.0f - bu) - bv;
+    float tu = (t[0] * w + t[2] * bu) + t[4] * bv;
+    float tv = (t[1] * w + t[3] * bu) + t[5] * bv;
+    return base * sample_texture(gload(sc.textures + m.texture), sc.texels, tu, tv);
+}
+
 // ---------------------------------------------------------------- intersection
 // rendering/taichi/intersect.py:17-25
 __device__ __forceinline__ f3 inv_dir(f3 d) {

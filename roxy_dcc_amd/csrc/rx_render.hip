@@ -39,6 +39,14 @@ constexpr int MAX_JOBS = 4096;
 // blockIdx % NPART, and are folded into their destination once per pass.
 constexpr int NPART = 64;
 
+// Adaptive-stop error sums: |change of the running mean| per sample index,
+// summed over pixels and channels in fixed point (2^22 units per colour
+// unit, each term capped at 2^14) so the sum, and the stop decision, do not
+// depend on the order of the atomics; EPART partial slots per job.
+constexpr int EPART = 16;
+constexpr double ERR_SCALE = 4194304.0;
+constexpr float ERR_CAP = 16384.0f;
+
 // Path state and queues, SoA over the workspace.
 struct ws_view {
     rx_job* jobs;
@@ -66,11 +74,16 @@ struct ws_view {
     // slot per NPART blocks (blockIdx % NPART), folded once per pass
     unsigned long long* ctr_part;   // NPART x 16 (an rx_counters per 128 bytes)
     unsigned long long* ray_part;   // NPART x n_jobs
+    // samples of this pass each job accumulates (k_decide), and the
+    // adaptive-stop error sums: EPART x n_jobs x err_stride, fixed point
+    int32_t* ns_eff;
+    unsigned long long* err_part;
+    int err_stride;
     // traversal stack spill area (entries past the LDS part)
     int2* spill;
 };
 
-void carve(rx_arena& ar, int n_jobs, long long P, bool deep, ws_view& w) {
+void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w) {
     w.jobs = ar.take<rx_job>(n_jobs);
     w.spill = ar.take<int2>(trav_spill_entries(deep));
     w.path_off = ar.take<int32_t>(n_jobs + 1);
@@ -79,6 +92,9 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, ws_view& w) {
     w.q_off = ar.take<int32_t>(n_jobs + 1);
     w.ctr_part = ar.take<unsigned long long>(NPART * 16);
     w.ray_part = ar.take<unsigned long long>((size_t)NPART * n_jobs);
+    w.ns_eff = ar.take<int32_t>(n_jobs);
+    w.err_stride = err_stride;
+    w.err_part = ar.take<unsigned long long>((size_t)EPART * n_jobs * err_stride);
     float** pf[] = {&w.thr_x, &w.thr_y, &w.thr_z, &w.col_x, &w.col_y, &w.col_z, &w.ior,
                     &w.fn_x,  &w.fn_y,  &w.fn_z,  &w.fa_x,  &w.fa_y,  &w.fa_z,  &w.fd};
     for (float** f : pf) *f = ar.take<float>(P);
@@ -324,8 +340,12 @@ __global__ __launch_bounds__(PB) void k_pass_begin(ws_view w) {
 // k_advance clears them after every fold).
 __global__ void k_clear_ray_part(ws_view w) {
     long long n = (long long)NPART * w.n_jobs;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
-        w.ray_part[k] = 0ull;
+    long long ne = (long long)EPART * w.n_jobs * w.err_stride;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n || k < ne;
+         k += (long long)gridDim.x * blockDim.x) {
+        if (k < n) w.ray_part[k] = 0ull;
+        if (k < ne) w.err_part[k] = 0ull;
+    }
 }
 
 // End of a pass: moves every job's sample window, folds the job's ray
@@ -335,9 +355,9 @@ __global__ void k_advance(ws_view w, rx_counters* __restrict__ ctr) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < w.n_jobs) {
         rx_job& J = w.jobs[j];
-        int ns = pass_samples(J);
+        int ns = w.ns_eff[j];
         J.first_sample += ns;
-        J.samples_left -= ns;
+        J.samples_left = J.adaptive_stopped ? 0 : J.samples_left - ns;
         unsigned long long r = 0;
         for (int k = 0; k < NPART; ++k) {
             r += w.ray_part[(size_t)k * w.n_jobs + j];
@@ -742,23 +762,24 @@ __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
                 col = col + thr * sky;
             } else {
                 rx_material m;
-                f3 nn;
+                f3 nn, albedo;
                 if (kind == RX_KIND_BVH) {
                     const rx_tri_shade sh = gload(sc.tri_shade + id);
                     float u = hu, v = hv;
                     float ww = (1.0f - u) - v;
                     nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
                     m = gload(sc.materials + sh.mat);
+                    albedo = bvh_albedo(sc, m, id, sh.has_uv, u, v, ld3(m.albedo));
                 } else {
                     const rx_prim pr = gload(sc.prims + id);
                     nn = prim_normal(pr, ro, rd, t);
                     m = gload(sc.prim_materials + id);
+                    albedo = ld3(m.albedo);
                 }
                 // trace.py:161-165
                 bool front = dot(nn, rd) < 0.0f;
                 if (!front) nn = -nn;
                 hit_p = ro + rd * t;
-                f3 albedo = ld3(m.albedo);
                 if (depth == 0) {
                     w.fn_x[p] = nn.x;
                     w.fn_y[p] = nn.y;
@@ -1054,6 +1075,113 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     }
 }
 
+// ---------------------------------------------------------------- adaptive stop
+// A path's colour sample as it is accumulated (trace.py:262-266 clamp).
+__device__ __forceinline__ f3 clamped_sample(const ws_view& w, const rx_render_params& prm, long long p) {
+    f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+    if (prm.sample_clamp > 0.0f) {
+        float lim = prm.sample_clamp;
+        smp = mk(fminf(fmaxf(smp.x, 0.0f), lim), fminf(fmaxf(smp.y, 0.0f), lim), fminf(fmaxf(smp.z, 0.0f), lim));
+    }
+    return smp;
+}
+
+__device__ __forceinline__ unsigned long long err_units(float d) {
+    return __double2ull_rn((double)fminf(d, ERR_CAP) * ERR_SCALE);
+}
+
+// Error sums of rendering/ray_tracer.py:534 for every sample of this pass of
+// the adaptive jobs: the change of each pixel's running mean (the same
+// arithmetic as k_accumulate) summed over channels and pixels per sample
+// index, into the job's partial slots.  Accumulators are only read.
+__global__ __launch_bounds__(BS) void k_adapt_err(ws_view w) {
+    const int stride = gridDim.x * BS;
+    const int lane = threadIdx.x & 63;
+    for (long long base = (long long)xcd_block() * BS; base < w.NP; base += stride) {
+        long long q = base + threadIdx.x;
+        int jx = 0, ns = 0, k = 0, npix = 1, first = 0;
+        f3 c = mk(0.0f, 0.0f, 0.0f);
+        const rx_job* J = nullptr;
+        if (q < w.NP) {
+            jx = w.n_jobs == 1 ? 0 : owner(w.pix_off, w.n_jobs, q);
+            J = &w.jobs[jx];
+            if (J->params.flags & RX_FLAG_ADAPTIVE) {
+                ns = pass_samples(*J);
+                npix = J->camera.width * J->camera.height;
+                k = (int)(q - w.pix_off[jx]);
+                first = J->first_sample;
+                if (ns > 0) c = ld3(J->accum.color + 3 * slot_pixel(J->camera, k));
+            }
+        }
+        // every lane walks the wave's longest sample range (wave-wide sums)
+        int wave_ns = ns;
+        for (int off = 32; off > 0; off >>= 1) wave_ns = max(wave_ns, __shfl_xor(wave_ns, off, 64));
+        int j0 = __shfl(jx, 0, 64);
+        bool uniform = __ballot(jx != j0) == 0ull;
+        unsigned long long* part = w.err_part + ((size_t)(blockIdx.x & (EPART - 1)) * w.n_jobs) * w.err_stride;
+        for (int s = 0; s < wave_ns; ++s) {
+            unsigned long long units = 0;
+            if (s < ns) {
+                long long p = (long long)w.path_off[jx] + (long long)s * npix + k;
+                f3 smp = clamped_sample(w, J->params, p);
+                int frame = first + s;
+                if (frame == 0) {
+                    c = smp;
+                } else {
+                    float f = (float)frame, f1 = (float)(frame + 1);
+                    f3 nc = (c * f + smp) / f1;
+                    units = err_units(fabsf(nc.x - c.x)) + err_units(fabsf(nc.y - c.y)) + err_units(fabsf(nc.z - c.z));
+                    c = nc;
+                }
+            }
+            if (uniform) {
+                unsigned long long t = wave_sum_u64(units);
+                if (lane == 0 && t) atomicAdd(part + (size_t)j0 * w.err_stride + s, t);
+            } else if (units) {
+                atomicAdd(part + (size_t)jx * w.err_stride + s, units);
+            }
+        }
+    }
+}
+
+// Samples of this pass each job accumulates (one thread per job): all of
+// them, or for an adaptive job the samples up to the first n at which
+// rendering/ray_tracer.py:531-541 stops: n < requested total,
+// n >= min_samples, (n - min_samples) % check_interval == 0 and error <=
+// threshold, the error after n >= 2 samples being the mean over pixels and
+// channels of |mean_n - mean_(n-1)|.  Clears the error sums it read.
+__global__ void k_decide(ws_view w) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= w.n_jobs) return;
+    rx_job& J = w.jobs[j];
+    int ns = pass_samples(J), eff = ns;
+    if ((J.params.flags & RX_FLAG_ADAPTIVE) && ns > 0) {
+        const rx_render_params& prm = J.params;
+        const long long total = (long long)J.first_sample + J.samples_left;
+        const double denom = 3.0 * (double)J.camera.width * (double)J.camera.height;
+        bool stop = false;
+        for (int s = 0; s < ns; ++s) {
+            unsigned long long sum = 0;
+            for (int e = 0; e < EPART; ++e) {
+                unsigned long long* cell = w.err_part + ((size_t)e * w.n_jobs + j) * w.err_stride + s;
+                sum += *cell;
+                *cell = 0ull;
+            }
+            long long n = (long long)J.first_sample + s + 1;
+            if (stop || n < 2) continue;
+            double err = (double)sum / ERR_SCALE / denom;
+            J.adaptive_error = (float)err;
+            if (n < total && n >= prm.adaptive_min_samples &&
+                (n - prm.adaptive_min_samples) % prm.adaptive_check_interval == 0 && err <= (double)prm.adaptive_threshold) {
+                eff = s + 1;
+                stop = true;
+                J.adaptive_stopped = 1;
+            }
+        }
+    }
+    w.ns_eff[j] = eff;
+}
+
 // ---------------------------------------------------------------- accumulate
 // rendering/taichi/trace.py:262-283: clamp, running mean of colour and the
 // first-hit guides in sample order, display = sqrt(min(acc, 1)); the
@@ -1072,19 +1200,14 @@ __global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
             int npix = J.camera.width * J.camera.height;
             int k = (int)(q - w.pix_off[jx]);  // slot within a sample
             int pix = slot_pixel(J.camera, k);
-            int ns = pass_samples(J);
+            int ns = w.ns_eff[jx];
             if (ns > 0) {
                 f3 c = ld3(acc.color + 3 * pix), nn = ld3(acc.normal + 3 * pix), aa = ld3(acc.albedo + 3 * pix);
                 float dd = acc.depth[pix];
                 for (int s = 0; s < ns; ++s) {
                     long long p = (long long)w.path_off[jx] + (long long)s * npix + k;
                     int frame = J.first_sample + s;
-                    f3 smp = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
-                    if (prm.sample_clamp > 0.0f) {
-                        float lim = prm.sample_clamp;
-                        smp = mk(fminf(fmaxf(smp.x, 0.0f), lim), fminf(fmaxf(smp.y, 0.0f), lim),
-                                 fminf(fmaxf(smp.z, 0.0f), lim));
-                    }
+                    f3 smp = clamped_sample(w, prm, p);
                     f3 ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
                     f3 as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
                     float ds = w.fd[p];
@@ -1206,6 +1329,8 @@ struct pass_plan {
     bool count;              // traversal counters
     bool prim_prepass;       // closest-primitive scan as its own pass
     int trav_per_cu;         // resident traversal blocks per CU (0 = default)
+    int max_spp;             // most samples per pass of any job
+    bool adaptive;           // some job has RX_FLAG_ADAPTIVE
 };
 
 // Scenes with more analytic primitives than this get their closest-
@@ -1246,6 +1371,8 @@ int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_
                 RX_STAGE(tm, 3, (k_connect<SBS, false><<<gt, SBS, 0, s>>>(w, d)));
         }
     }
+    if (pl.adaptive) RX_STAGE(tm, 4, (k_adapt_err<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
+    RX_STAGE(tm, 4, (k_decide<<<(w.n_jobs + 255) / 256, 256, 0, s>>>(w)));
     RX_STAGE(tm, 4, (k_accumulate<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
     RX_STAGE(tm, 4, (k_advance<<<(w.n_jobs + 255) / 256, 256, 0, s>>>(w, ctr)));
     return RX_OK;
@@ -1266,6 +1393,12 @@ int check_job(const rx_job& j, int idx) {
     RX_REQUIRE(p.max_depth >= 0, RX_ERR_INVALID, "job %d: max_depth must be >= 0", idx);
     RX_REQUIRE(p.trav_blocks_per_cu >= 0 && p.trav_blocks_per_cu <= TRAV_BLOCKS_PER_CU_DEEP, RX_ERR_INVALID,
                "job %d: trav_blocks_per_cu must be in 0..%d", idx, TRAV_BLOCKS_PER_CU_DEEP);
+    if (p.flags & RX_FLAG_ADAPTIVE) {
+        // rendering/ray_tracer.py:314-319
+        RX_REQUIRE(p.adaptive_min_samples >= 1, RX_ERR_INVALID, "adaptive_min_samples must be at least 1");
+        RX_REQUIRE(p.adaptive_threshold >= 0.0f, RX_ERR_INVALID, "adaptive_threshold must be non-negative");
+        RX_REQUIRE(p.adaptive_check_interval >= 1, RX_ERR_INVALID, "adaptive_check_interval must be at least 1");
+    }
     RX_REQUIRE(p.direct_light_mode == 0 || p.direct_light_mode == 1, RX_ERR_INVALID,
                "direct_light_mode must be 'one' or 'all'");
     RX_REQUIRE(j.accum.color && j.accum.normal && j.accum.albedo && j.accum.depth, RX_ERR_INVALID,
@@ -1276,6 +1409,9 @@ int check_job(const rx_job& j, int idx) {
                "job %d: scene has primitives but no primitive arrays", idx);
     RX_REQUIRE(sc.n_lights == 0 || sc.lights, RX_ERR_INVALID, "job %d: scene has lights but lights is NULL", idx);
     RX_REQUIRE(rx_scene_aligned(sc), RX_ERR_INVALID, "job %d: scene arrays must be 16-byte aligned", idx);
+    RX_REQUIRE(sc.n_textures >= 0, RX_ERR_INVALID, "job %d: negative texture count", idx);
+    RX_REQUIRE(sc.n_textures == 0 || (sc.textures && sc.texels && (sc.n_tris == 0 || sc.tri_uv)), RX_ERR_INVALID,
+               "job %d: a textured scene needs textures, texels and per-triangle UVs", idx);
     if (sc.n_tris > 0) {
         RX_REQUIRE(sc.nodes && sc.header && sc.tris && sc.tri_shade && sc.materials, RX_ERR_INVALID,
                    "job %d: scene has triangles but no BVH/shading arrays", idx);
@@ -1290,7 +1426,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false, false, 0};
+    pass_plan plan{0, 0, false, false, false, 0, 0, false};
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -1306,6 +1442,8 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
         if (j.scene.n_tris > 0 && use_deep_stack(j.scene.bvh_depth)) plan.deep = true;
         if (j.params.flags & RX_FLAG_SMALL_BLOCKS) plan.deep = true;
         if (j.params.trav_blocks_per_cu > plan.trav_per_cu) plan.trav_per_cu = j.params.trav_blocks_per_cu;
+        if (j.params.samples_per_pass > plan.max_spp) plan.max_spp = j.params.samples_per_pass;
+        if (j.params.flags & RX_FLAG_ADAPTIVE) plan.adaptive = true;
         if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
         if (j.scene.n_prims > 0 && (j.scene.n_prims > PRIM_INLINE_MAX || (j.params.flags & RX_FLAG_PRIM_PREPASS)))
             plan.prim_prepass = true;
@@ -1319,20 +1457,21 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
     return RX_OK;
 }
 
-size_t workspace_bytes_for(int n_jobs, long long P, bool deep) {
+size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride) {
     rx_arena ar{nullptr, 0, 0};
     ws_view w;
-    carve(ar, n_jobs, P, deep, w);
+    carve(ar, n_jobs, P, deep, err_stride, w);
     return ar.off + 256;
 }
 
-int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, bool deep, ws_view& w) {
-    size_t need = workspace_bytes_for(n_jobs, P, deep);
+int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, bool deep, int err_stride,
+               ws_view& w) {
+    size_t need = workspace_bytes_for(n_jobs, P, deep, err_stride);
     RX_REQUIRE(d_workspace && bytes >= need, RX_ERR_WORKSPACE, "render workspace too small: need %zu bytes, have %zu",
                need, bytes);
     rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, 0};
     ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
-    carve(ar, n_jobs, P, deep, w);
+    carve(ar, n_jobs, P, deep, err_stride, w);
     w.NP = NP;
     return RX_OK;
 }
@@ -1357,7 +1496,7 @@ extern "C" int rx_render_workspace_size(int width, int height, int samples_per_p
                "width, height and samples_per_pass must be positive");
     long long P = (long long)width * height * samples_per_pass;
     RX_REQUIRE(P <= 0x7fffffffLL, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
-    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth));
+    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth), samples_per_pass);
     return RX_OK;
 }
 
@@ -1373,6 +1512,8 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     j.first_sample = first_sample;
     j.samples_left = n_samples;
     j.rays = 0;
+    j.adaptive_error = -1.0f;
+    j.adaptive_stopped = 0;
     long long P = 0, NP = 0;
     pass_plan pl;
     int32_t poff[2], qoff[2];
@@ -1382,7 +1523,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     j.pixel_offset = 0;
     if (n_samples == 0) return RX_OK;
     ws_view w;
-    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, w);
+    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, pl.max_spp, w);
     if (st) return st;
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
@@ -1411,7 +1552,7 @@ extern "C" int rx_batch_workspace_size(const rx_job* jobs, int n_jobs, size_t* b
     pass_plan pl;
     int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, nullptr, nullptr);
     if (st) return st;
-    *bytes = workspace_bytes_for(n_jobs, P, pl.deep);
+    *bytes = workspace_bytes_for(n_jobs, P, pl.deep, pl.max_spp);
     return RX_OK;
 }
 
@@ -1428,12 +1569,14 @@ extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace
     pass_plan pl;
     int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, poff, qoff);
     ws_view w;
-    if (!st) st = carve_into(d_workspace, workspace_bytes, n_jobs, P, NP, pl.deep, w);
+    if (!st) st = carve_into(d_workspace, workspace_bytes, n_jobs, P, NP, pl.deep, pl.max_spp, w);
     if (!st) {
         for (int i = 0; i < n_jobs; ++i) {
             hj[i] = jobs[i];
             hj[i].path_offset = poff[i];
             hj[i].pixel_offset = qoff[i];
+            hj[i].adaptive_error = -1.0f;
+            hj[i].adaptive_stopped = 0;
         }
         hipError_t e = hipMemcpy(w.jobs, hj, sizeof(rx_job) * n_jobs, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(w.path_off, poff, sizeof(int32_t) * (n_jobs + 1), hipMemcpyHostToDevice);

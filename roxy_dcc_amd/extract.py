@@ -31,18 +31,43 @@ class ExtractResult(dict):
     flat: FlatScene = None
 
 
-def _material_data(mat):
+# Texture cache capacity (rendering/taichi/fields.py:9-10).
+MAX_TEXTURES = 8
+MAX_TEXTURE_SIZE = 2048
+
+
+class _Textures:
+    """The texture list of extractor.py:93-117: one slot per distinct
+    (path, flip_v, identity of an in-memory texture), first-use order."""
+
+    def __init__(self):
+        self.items = []
+        self.index = {}
+
+    def idx(self, mat):
+        texture = getattr(mat, "_albedo_texture", None)
+        if texture is None:
+            return -1
+        key = (texture.path, bool(texture.flip_v), id(texture) if texture.path is None else None)
+        if key in self.index:
+            return self.index[key]
+        if len(self.items) >= MAX_TEXTURES:
+            raise ValueError(f"Scene has more than {MAX_TEXTURES} image textures; increase MAX_TEXTURES")
+        self.index[key] = len(self.items)
+        self.items.append(texture)
+        return self.index[key]
+
+
+def _material_data(mat, textures):
     mt = mat.taichi_type_id()
     params = mat.taichi_params()
-    if getattr(mat, "_albedo_texture", None) is not None:
-        raise NotImplementedError("image textures are not supported by this backend yet")
     return {
         "mat_type": mt,
         "albedo": [float(c) for c in mat._albedo],
         "roughness": params[0] if mt in (1, 4) else 0.0,
         "ior": params[0] if mt == 2 else 1.0,
         "emission": params[0] if mt == 3 else 0.0,
-        "texture_idx": -1,
+        "texture_idx": textures.idx(mat),
     }
 
 
@@ -107,6 +132,7 @@ def flatten_scene(world) -> FlatScene:
     prim_slots = []
     tri_dicts = []
     batches = []
+    textures = _Textures()
 
     def collect(obj):
         if not obj.renderable:
@@ -132,14 +158,14 @@ def flatten_scene(world) -> FlatScene:
                         "n2": xn(tri._n2 if tri._n2 is not None else face),
                         "uv0": _uv2(tri._uv0), "uv1": _uv2(tri._uv1), "uv2": _uv2(tri._uv2),
                         "has_uv": int(tri._uv0 is not None and tri._uv1 is not None and tri._uv2 is not None),
-                        **_material_data(mat),
+                        **_material_data(mat, textures),
                     })
             elif _is_indexed(geo):
                 arrays = indexed_arrays_native(geo, obj.world_matrix, obj.world_inverse_transpose_matrix)
                 by_group = {}
                 for gi, group in enumerate(arrays["groups"]):
                     mat = shape.material_for_group(group) or _default_material()
-                    by_group[gi] = _material_data(mat)
+                    by_group[gi] = _material_data(mat, textures)
                 batches.append(TriangleBatch.from_indexed_arrays(arrays, by_group))
             else:
                 exported = obj.taichi_export()
@@ -155,7 +181,7 @@ def flatten_scene(world) -> FlatScene:
         collect(obj)
     if tri_dicts:
         batches.append(TriangleBatch.from_dicts(tri_dicts))
-    return FlatScene(batches=batches, prims=PrimitiveSlots.from_dicts(prim_slots))
+    return FlatScene(batches=batches, prims=PrimitiveSlots.from_dicts(prim_slots), textures=textures.items)
 
 
 def extract_scene(world, device="cuda", max_leaf=None, method=None) -> ExtractResult:

@@ -100,11 +100,17 @@ def camera_struct(cam, width, height):
 
 
 def params_struct(max_depth, use_sky, mode_id, dl_max_depth, sample_clamp, bg, seed, samples_per_pass, flags=0,
-                  trav_blocks_per_cu=0):
+                  trav_blocks_per_cu=0, adaptive=None):
+    """adaptive: None, or (threshold, min_samples, check_interval) to set
+    RX_FLAG_ADAPTIVE."""
     seed = int(seed) & ((1 << 64) - 1)
+    thr, amin, aint = 0.0, 1, 1
+    if adaptive is not None:
+        thr, amin, aint = adaptive
+        flags |= nat.RX_FLAG_ADAPTIVE
     return nat.RxRenderParams(int(max_depth), int(use_sky), int(mode_id), int(dl_max_depth), float(sample_clamp),
                               (C.c_float * 3)(*[float(x) for x in bg]), seed & 0xFFFFFFFF, seed >> 32, int(flags),
-                              int(samples_per_pass), int(trav_blocks_per_cu))
+                              int(samples_per_pass), int(trav_blocks_per_cu), float(thr), int(amin), int(aint))
 
 
 class TaichiRenderer:
@@ -113,7 +119,21 @@ class TaichiRenderer:
     def __init__(self, world, image, viewport, samples=16, max_depth=4, direct_light_mode="one", denoise=False,
                  denoise_radius=1, denoise_sigma=0.08, denoise_amount=0.8, sample_clamp=10.0,
                  direct_light_max_depth=1, startup_progress=None, *, seed=0, samples_per_pass=None,
-                 denoise_passes=1, device="cuda", verbose=True):
+                 denoise_passes=1, adaptive_sampling=False, adaptive_min_samples=4, adaptive_threshold=0.002,
+                 adaptive_check_interval=1, device="cuda", verbose=True):
+        # adaptive stop as the CPU tracer has it (rendering/ray_tracer.py:292-323)
+        if adaptive_min_samples < 1:
+            raise ValueError("adaptive_min_samples must be at least 1")
+        if adaptive_threshold < 0:
+            raise ValueError("adaptive_threshold must be non-negative")
+        if adaptive_check_interval < 1:
+            raise ValueError("adaptive_check_interval must be at least 1")
+        self._adaptive_sampling = bool(adaptive_sampling)
+        self._adaptive_min_samples = int(adaptive_min_samples)
+        self._adaptive_threshold = float(adaptive_threshold)
+        self._adaptive_check_interval = int(adaptive_check_interval)
+        self._adaptive_stopped = False
+        self._last_adaptive_error = None
         self._world = world
         self._image = image
         self._viewport = viewport
@@ -183,38 +203,88 @@ class TaichiRenderer:
         stats = extract_scene(world, device=self._device)
         self.scene = stats.scene
         spp = self._samples_per_pass or default_samples_per_pass(W, H, self._samples)
-        self.frame = DeviceFrame(W, H, device=self._device)
-        self.integrator = Integrator(self.scene, W, H, spp, device=self._device)
-        camera = camera_struct(cam, W, H)
-        params = params_struct(self._max_depth, use_sky, self._direct_light_mode_id, self._direct_light_max_depth,
-                               self._sample_clamp, bg, self._seed, spp)
         if self._startup_progress:
             self._startup_progress.close()
             self._startup_progress = None
-        done = 0
-        while done < self._samples:
-            if self._viewport and getattr(self._viewport, "should_close", False):
-                break
-            n = min(spp, self._samples - done)
-            self.integrator.render(self.frame, camera, params, done, n)
-            done += n
-            if self._viewport:
-                self._copy_display_pixels()
-                self._viewport.update(self._image)
-                self._viewport.poll_events()
+        self._adaptive_stopped = False
+        self._last_adaptive_error = None
+        if self._adaptive_sampling:
+            done, rays = self._render_adaptive(W, H, spp, use_sky, bg)
+        else:
+            done, rays = self._render_fixed(W, H, spp, use_sky, bg)
         torch.cuda.synchronize()
         self._copy_display_pixels(apply_denoise=True)
-        counts = self.integrator.counts()
-        self._last_ray_count = counts["rays"]
+        self._last_ray_count = rays
         s = self.scene.stats()
         self._last_stats = RenderStats(width=W, height=H, samples_requested=self._samples, samples_rendered=done,
-                                       max_depth=self._max_depth, rays_cast=counts["rays"],
+                                       max_depth=self._max_depth, rays_cast=rays,
                                        elapsed_seconds=time.perf_counter() - t0,
                                        primitive_count=s["primitive_count"], bvh_nodes=s["bvh_nodes"],
                                        bvh_triangles=s["bvh_triangles"], bvh_materials=s["bvh_materials"],
-                                       bvh_leaf_size=s["bvh_leaf_size"])
+                                       bvh_leaf_size=s["bvh_leaf_size"], adaptive_sampling=self._adaptive_sampling,
+                                       adaptive_stopped=self._adaptive_stopped,
+                                       adaptive_threshold=self._adaptive_threshold,
+                                       adaptive_error=self._last_adaptive_error,
+                                       adaptive_min_samples=self._adaptive_min_samples)
         if self._verbose:
             print("\n" + self._last_stats.format_report())
+
+    def _viewport_tick(self):
+        if self._viewport:
+            self._copy_display_pixels()
+            self._viewport.update(self._image)
+            self._viewport.poll_events()
+
+    def _closing(self):
+        return bool(self._viewport and getattr(self._viewport, "should_close", False))
+
+    def _render_fixed(self, W, H, spp, use_sky, bg):
+        """All requested samples in passes of `spp` (rx_render)."""
+        self.frame = DeviceFrame(W, H, device=self._device)
+        self.integrator = Integrator(self.scene, W, H, spp, device=self._device)
+        camera = camera_struct(self._camera, W, H)
+        params = params_struct(self._max_depth, use_sky, self._direct_light_mode_id, self._direct_light_max_depth,
+                               self._sample_clamp, bg, self._seed, spp)
+        done = 0
+        while done < self._samples and not self._closing():
+            n = min(spp, self._samples - done)
+            self.integrator.render(self.frame, camera, params, done, n)
+            done += n
+            self._viewport_tick()
+        return done, self.integrator.counts()["rays"]
+
+    def _render_adaptive(self, W, H, spp, use_sky, bg):
+        """The adaptive stop of the CPU tracer (rendering/ray_tracer.py:531-541)
+        decided on the device after every sample: a one-job batch whose record
+        reports the samples taken, the rays they cast and the last error."""
+        from .batch import Batch, RenderJob
+
+        job = RenderJob(self.scene, self._camera, W, H, self._samples, max_depth=self._max_depth,
+                        direct_light_mode=self._direct_light_mode, direct_light_max_depth=self._direct_light_max_depth,
+                        sample_clamp=self._sample_clamp, use_sky=bool(use_sky), background=tuple(bg),
+                        seed=self._seed, samples_per_pass=spp,
+                        adaptive=(self._adaptive_threshold, self._adaptive_min_samples,
+                                  self._adaptive_check_interval))
+        job.frame = self.frame = DeviceFrame(W, H, device=self._device)
+        self.integrator = None
+        batch = Batch([job], use_graph=False, device=self._device)
+        rec = None
+        for _ in range(job.passes_needed):
+            if self._closing():
+                break
+            batch.run(1)
+            rec = batch.read_jobs()[0]
+            self._viewport_tick()
+            if rec.samples_left <= 0:
+                break
+        batch.close()
+        if rec is None:
+            return 0, 0
+        self._adaptive_stopped = bool(rec.adaptive_stopped)
+        self._last_adaptive_error = None if rec.adaptive_error < 0 else float(rec.adaptive_error)
+        # the job's rays are those of the samples it kept (a pass's samples
+        # past the stop are traced but neither accumulated nor counted)
+        return int(rec.first_sample), int(rec.rays)
 
     @property
     def last_ray_count(self):

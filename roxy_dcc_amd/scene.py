@@ -19,11 +19,14 @@ from .bvh import MATERIAL_DTYPE, GPUBVHBuilder, TriangleBatch
 
 PRIM_DTYPE = np.dtype([("a", "<f4", 4), ("b", "<f4", 4), ("c", "<f4", 3), ("type", "<i4")])
 LIGHT_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("albedo", "<f4", 3), ("emission", "<f4")])
+TEXTURE_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("flip_v", "<i4"), ("pad0", "<i4"),
+                          ("offset", "<i8"), ("pad1", "<i8")])
 assert PRIM_DTYPE.itemsize == nat.PRIM_BYTES and LIGHT_DTYPE.itemsize == nat.LIGHT_BYTES
 
 # capacity limits of the reference fields (rendering/taichi/fields.py:3, :11)
 MAX_OBJECTS = 10_000_000
 MAX_LIGHTS = 4_096
+MAX_TEXTURE_SIZE = 2048        # rendering/taichi/fields.py: textures are downsampled to fit
 
 
 @dataclass
@@ -132,6 +135,7 @@ class FlatScene:
 
     batches: list = field(default_factory=list)
     prims: PrimitiveSlots = field(default_factory=PrimitiveSlots.empty)
+    textures: list = field(default_factory=list)     # ImageTexture, indexed by material texture_idx
 
     @property
     def triangle_count(self):
@@ -169,6 +173,23 @@ class DeviceScene:
         self.d_prims = dev(flat.prims.records())
         self.d_prim_mats = dev(flat.prims.material_records())
         self.d_lights = dev(lights.records())
+        # texture cache (extractor.py:53-81): one u8 RGB pool, a 32-byte
+        # record per texture, and leaf-order UVs for the triangles
+        self.n_textures = len(flat.textures)
+        self.d_textures = self.d_texels = self.d_tri_uv = None
+        if self.n_textures:
+            recs = np.zeros(self.n_textures, dtype=TEXTURE_DTYPE)
+            chunks, off = [], 0
+            for i, tex in enumerate(flat.textures):
+                px = np.ascontiguousarray(tex.load_pixels_u8(max_size=MAX_TEXTURE_SIZE), dtype=np.uint8)
+                h, w = px.shape[:2]
+                recs[i] = (w, h, 1 if tex.flip_v else 0, 0, off, 0)
+                chunks.append(px.reshape(-1))
+                off += px.size
+            self.d_texels = torch.from_numpy(np.concatenate(chunks)).to(device)
+            self.d_textures = dev(recs)
+            if self.builder.triangle_count:
+                self.d_tri_uv = self.builder.pack_uvs()
         self._struct = None
 
     @property
@@ -195,8 +216,11 @@ class DeviceScene:
                 d["shade"].data_ptr() if has_bvh else None,
                 d["materials"].data_ptr() if has_bvh else None,
                 self.d_prims.data_ptr(), self.d_prim_mats.data_ptr(), self.d_lights.data_ptr(),
+                self.d_tri_uv.data_ptr() if self.d_tri_uv is not None else None,
+                self.d_textures.data_ptr() if self.d_textures is not None else None,
+                self.d_texels.data_ptr() if self.d_texels is not None else None,
                 b.node_count, b.triangle_count, len(b.mat_palette), self.n_prims, self.n_lights,
-                b.depth)
+                b.depth, self.n_textures, 0)
         return self._struct
 
     def stats(self):
@@ -212,4 +236,6 @@ class DeviceScene:
     def hbm_bytes(self) -> int:
         """Device bytes this scene keeps resident."""
         total = sum(t.numel() * t.element_size() for t in self.builder.d.values())
-        return total + sum(t.numel() for t in (self.d_prims, self.d_prim_mats, self.d_lights))
+        extra = [t for t in (self.d_prims, self.d_prim_mats, self.d_lights, self.d_textures, self.d_texels)
+                 if t is not None]
+        return total + sum(t.numel() * t.element_size() for t in extra)

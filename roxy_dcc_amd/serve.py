@@ -37,9 +37,14 @@ class Request:
     arrival: float            # seconds after the server clock starts
     max_depth: int = 6
     seed: int = 0
+    # error target instead of a fixed sample count: (threshold, min_samples,
+    # check_interval) of the adaptive stop (rendering/ray_tracer.py:531-541),
+    # `samples` then being the cap
+    adaptive: tuple = None
     # filled in by the server
     admitted: float = None
     finished: float = None
+    samples_done: int = 0
     frame: object = field(default=None, repr=False)
     rays: int = 0
 
@@ -150,7 +155,7 @@ class RenderServer:
             pinned = [r.scene_key for r, _ in self.active]
             scene, cam = self.cache.get(req.scene_key, pinned=pinned)
             job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
-                            samples_per_pass=1, tag=req)
+                            samples_per_pass=1, adaptive=req.adaptive, tag=req)
             req.admitted = now
             self.active.append((req, job))
             used += need
@@ -181,6 +186,7 @@ class RenderServer:
             for (req, job), rec in zip(self.active, recs):
                 job.first_sample = rec.first_sample
                 req.rays += int(rec.rays)
+                req.samples_done = int(rec.first_sample)
                 if rec.samples_left <= 0:
                     torch.cuda.synchronize()
                     req.frame = job.frame.color.cpu().numpy()
@@ -207,13 +213,14 @@ def serial_loop(cache: SceneCache, requests, paths_per_pass=1 << 22, clock=time.
         scene, cam = cache.get(req.scene_key)
         spp = int(max(1, min(req.samples, paths_per_pass // max(1, req.width * req.height))))
         job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
-                        samples_per_pass=spp, tag=req)
+                        samples_per_pass=spp, adaptive=req.adaptive, tag=req)
         b = Batch([job], use_graph=False)
         b.run()
         rec = b.read_jobs()[0]
         b.close()
         torch.cuda.synchronize()
         req.rays = int(rec.rays)
+        req.samples_done = int(rec.first_sample)
         req.frame = job.frame.color.cpu().numpy()
         req.finished = clock() - t0
         out.append(req)

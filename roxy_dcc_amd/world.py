@@ -70,6 +70,62 @@ def _unit(v):
     return v / n if n > 0 else np.zeros(3)
 
 
+# ------------------------------------------------------------------ textures
+class ImageTexture:
+    """RGB image texture, repeating UVs, bilinear (scene/textures.py:6-70).
+    Image files are decoded with pygame, as in the reference; an in-memory
+    array needs nothing else."""
+
+    def __init__(self, path=None, pixels=None, flip_v=True):
+        if path is None and pixels is None:
+            raise ValueError("ImageTexture needs either a path or pixel array")
+        self.path = str(path) if path is not None else None
+        self._pixels = None if pixels is None else self._coerce_pixels(pixels)
+        self.flip_v = flip_v
+
+    @classmethod
+    def from_array(cls, pixels, flip_v=True):
+        return cls(pixels=pixels, flip_v=flip_v)
+
+    @staticmethod
+    def _coerce_pixels(pixels):
+        arr = np.asarray(pixels, dtype=np.float32)
+        if arr.ndim != 3 or arr.shape[2] < 3:
+            raise ValueError("texture pixels must be an HxWx3 array")
+        arr = arr[:, :, :3]
+        if arr.size and arr.max() > 1.0:
+            arr = arr / 255.0
+        return np.clip(arr, 0.0, 1.0)
+
+    def _load_pixels(self):
+        if self._pixels is not None:
+            return self._pixels
+        try:
+            import pygame
+        except ImportError as exc:
+            raise RuntimeError("pygame is required to load image textures") from exc
+        surface = pygame.image.load(self.path)
+        pixels = pygame.surfarray.array3d(surface).astype(np.float32) / 255.0
+        self._pixels = np.transpose(pixels[:, :, :3], (1, 0, 2))
+        return self._pixels
+
+    def load_pixels_u8(self, max_size=None):
+        """uint8 HxWx3 for the texture cache (scene/textures.py:43-70)."""
+        if self.path is None:
+            return np.clip(self._load_pixels() * 255.0 + 0.5, 0, 255).astype(np.uint8)
+        try:
+            import pygame
+        except ImportError as exc:
+            raise RuntimeError("pygame is required to load image textures") from exc
+        surface = pygame.image.load(self.path)
+        width, height = surface.get_size()
+        if max_size is not None and max(width, height) > max_size:
+            scale = float(max_size) / float(max(width, height))
+            surface = pygame.transform.smoothscale(
+                surface, (max(1, int(round(width * scale))), max(1, int(round(height * scale)))))
+        return np.transpose(pygame.surfarray.array3d(surface)[:, :, :3], (1, 0, 2)).copy()
+
+
 # ------------------------------------------------------------------ materials
 class Material:
     type_id = 0

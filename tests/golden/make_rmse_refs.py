@@ -27,7 +27,7 @@ from roxy_dcc_amd.extract import flatten_scene  # noqa: E402
 from roxy_dcc_amd.renderer import TaichiRenderer  # noqa: E402
 from tests import helpers  # noqa: E402
 
-N_PIXELS = {"cornell": 4096, "mesh": 4096, "large": 16384}
+N_PIXELS = {"cornell": 4096, "mesh": 4096, "large": 65536}
 SPP = 4096
 PIXEL_SEED = 1234
 RENDER_SEED = 0x5EED

@@ -34,8 +34,10 @@ def prims_dict(prims):
 def flat_to_oracle(flat, builder="morton", max_leaf=8):
     tb = flat.triangles()
     palette, idx = bvh_oracle.material_palette(material_rows(tb))
+    textures = [(t.load_pixels_u8(max_size=2048), t.flip_v) for t in getattr(flat, "textures", [])]
     return orc.OracleScene(tb.v0, tb.v1, tb.v2, tb.n0, tb.n1, tb.n2, idx, palette, prims_dict(flat.prims),
-                           builder=builder, max_leaf=max_leaf)
+                           builder=builder, max_leaf=max_leaf, uvs=(tb.uv0, tb.uv1, tb.uv2), has_uv=tb.has_uv,
+                           textures=textures)
 
 
 def load(name, seed=None):
@@ -124,3 +126,37 @@ def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0
         "gpu_counts": counts, "cpu_counts": acc.counts.as_dict(),
         "extract": ex, "oracle_scene": osc,
     }
+
+
+def textured_floor_world(W=16, H=8, repeat=2.0, width=48, height=48, flip_v=True):
+    """A floor quad y = 0 over x, z in [0, 2] with uv = (x, z) * repeat / 2,
+    textured with a W x H map whose texels are linear in the texel
+    coordinates (R = 8 * column, G = 8 * row, B = 200), under a constant
+    background, seen from above.  Bilinear filtering reproduces a linear
+    map exactly, so the albedo at uv is a closed form."""
+    from roxy_dcc_amd.world import (Camera, Diffuse, ImageTexture, IndexedMesh, Mat4, SceneObject, Shape,
+                                    World)
+
+    cols, rows = np.meshgrid(np.arange(W), np.arange(H))
+    tex = np.stack([8 * cols, 8 * rows, np.full_like(cols, 200)], axis=-1).astype(np.uint8)
+    texture = ImageTexture.from_array(tex, flip_v=flip_v)
+    pos = np.array([[0, 0, 0], [2, 0, 0], [2, 0, 2], [0, 0, 2]], np.float64)
+    idx = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
+    uvs = pos[:, [0, 2]] * repeat / 2.0
+    mesh = IndexedMesh(pos, idx, uvs=uvs, tri_uv_idx=idx)
+    w = World(background_color=(0.5, 0.5, 0.5), use_sky=False)
+    w.add_object(SceneObject("floor", [Shape(mesh, Diffuse((1.0, 1.0, 1.0), albedo_texture=texture))], Mat4()))
+    w.add_camera(Camera(position=(1.0, 3.0, 1.0), forward=(0.0, -1.0, 0.0), up=(0.0, 0.0, 1.0), fov=60,
+                        width=width, height=height))
+    return w, tex, texture
+
+
+def textured_albedo_closed_form(p, W, H, repeat=2.0, flip_v=True):
+    """Albedo of the textured floor at world point p (N x 3)."""
+    u = p[:, 0] * repeat / 2.0
+    v = p[:, 2] * repeat / 2.0
+    u = u - np.floor(u)
+    v = v - np.floor(v)
+    if flip_v:
+        v = 1.0 - v
+    return np.stack([8 * u * (W - 1) / 255.0, 8 * v * (H - 1) / 255.0, np.full_like(u, 200 / 255.0)], axis=1)

@@ -320,3 +320,68 @@ def test_taichi_renderer_end_to_end():
     s = r.last_stats
     assert s.samples_rendered == 8 and s.bvh_triangles == 36 and s.bvh_nodes == 15
     assert r.last_ray_count == s.rays_cast > 64 * 64 * 8
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_adaptive_stop_matches_oracle(interval):
+    """The adaptive stop of rendering/ray_tracer.py:531-541 on the device:
+    the run stops at the same sample as the rule applied to the oracle's
+    per-sample errors (a threshold with a clear margin), mid-pass, and its
+    image, ray count and error equal the oracle's after that many samples."""
+    from oracle import oracle as orc
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd.world import Image
+
+    from roxy_dcc_amd.extract import flatten_scene
+
+    world, _, _, st = helpers.load("cornell")      # square camera, rendered at 32x32
+    W = H = 32
+    samples, amin = 48, 4
+
+    osc = helpers.flat_to_oracle(flatten_scene(world))
+    ocam = helpers.oracle_camera(world.active_camera, W, H)
+    oprm = orc.params(st["max_depth"], int(world._use_sky), 0, st["direct_light_max_depth"], 10.0,
+                      world._background_color, 9)
+    errs = orc.adaptive_errors(osc, ocam, oprm, samples, orc.Accum(W * H))
+    target = thr = None
+    for n in range(amin + 5, samples - 4):
+        if (n - amin) % interval:
+            continue
+        earlier = [errs[m] for m in range(amin, n) if (m - amin) % interval == 0]
+        if errs[n] < 0.97 * min(earlier):
+            target, thr = n, 0.5 * (errs[n] + min(earlier))
+            break
+    assert target is not None
+    assert orc.adaptive_stop(errs, samples, amin, thr, interval) == target
+    acc = orc.Accum(W * H)
+    orc.render(osc, ocam, oprm, 0, target, acc)
+
+    img = Image(W, H)
+    r = R.TaichiRenderer(world, img, None, samples=samples, max_depth=st["max_depth"], seed=9, samples_per_pass=8,
+                         adaptive_sampling=True, adaptive_min_samples=amin, adaptive_threshold=thr,
+                         adaptive_check_interval=interval, verbose=False)
+    r.render()
+    stats = r.last_stats
+    got = r.frame.color.cpu().numpy().reshape(-1, 3)
+    same = np.all(got == acc.color, axis=1).mean()
+    print(f"adaptive interval {interval}: stop {stats.samples_rendered} (oracle {target}), error "
+          f"{stats.adaptive_error:.6f} (oracle {errs[target]:.6f}), exact {same:.4f}, rays {r.last_ray_count} "
+          f"({acc.counts.rays})", flush=True)
+    assert stats.adaptive_stopped and stats.samples_rendered == target
+    assert same >= 0.99
+    assert r.last_ray_count == acc.counts.rays
+    assert abs(stats.adaptive_error - errs[target]) <= 1e-5 * max(errs[target], 1e-3)
+
+
+@pytest.mark.parametrize("flip_v", [True, False])
+def test_textured_render_matches_oracle(flip_v):
+    """Textured albedo (trace.py:96-136) in the wavefront shade stage: the
+    render and its first-hit albedo guide equal the oracle's."""
+    world, tex, _ = helpers.textured_floor_world(16, 8, flip_v=flip_v)
+    settings = dict(max_depth=3, direct_light_mode="one", direct_light_max_depth=1)
+    res = helpers.render_pair("floor", samples=3, world_tuple=(world, 48, 48, settings))
+    same = np.all(res["gpu_color"] == res["cpu_color"], axis=1).mean()
+    alb = np.all(res["gpu_albedo"] == res["cpu_albedo"], axis=1).mean()
+    print(f"textured floor flip_v={flip_v}: exact colour {same:.4f}, exact albedo {alb:.4f}", flush=True)
+    assert same >= 0.99 and alb >= 0.99
+    assert res["gpu_rays"] == res["cpu_rays"]

@@ -28,7 +28,8 @@ def _gpu_render(name, spp, seed):
     integ = R.Integrator(ex.scene, W, H, per)
     frame = R.DeviceFrame(W, H)
     cam = R.camera_struct(world.active_camera, W, H)
-    prm = R.params_struct(st["max_depth"], int(world._use_sky), 0, st["direct_light_max_depth"], 10.0,
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
                           world._background_color, seed, per)
     done = 0
     while done < spp:
@@ -38,7 +39,7 @@ def _gpu_render(name, spp, seed):
     return frame.color.cpu().numpy().reshape(-1, 3)
 
 
-@pytest.mark.parametrize("name", ["cornell", "mesh"])
+@pytest.mark.parametrize("name", ["cornell", "mesh", "large"])
 def test_rmse_against_fp64_oracle(name):
     ref = np.load(GOLDEN / f"rmse_{name}.npz")
     pix, want = ref["pixels"], ref["color"]

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.lib()
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.rx_version() == 1
+    assert lib.rx_version() == 2
 
 
 def test_morton_shape_closed_form_matches_recursion():
@@ -414,3 +414,32 @@ def test_native_flatten_rejects_bad_indices():
     mesh = IndexedMesh(np.zeros((3, 3)), np.array([[0, 1, 3]], np.int32))
     with pytest.raises(ValueError):
         indexed_arrays_native(mesh)
+
+
+def test_texture_cache_dedup_and_capacity():
+    """extractor.py:93-117: one cache slot per distinct texture (path, flip_v,
+    identity of an in-memory texture) in first-use order; more than
+    MAX_TEXTURES raises ValueError; u8 texels as scene/textures.py:50-52."""
+    import pytest
+
+    from roxy_dcc_amd.extract import MAX_TEXTURES, flatten_scene
+    from roxy_dcc_amd.world import Diffuse, ImageTexture, IndexedMesh, Mat4, SceneObject, Shape, World
+
+    def world_with(textures):
+        w = World()
+        for i, t in enumerate(textures):
+            mesh = IndexedMesh(np.eye(3), np.array([[0, 1, 2]], np.int32), uvs=np.zeros((3, 2)),
+                               tri_uv_idx=np.array([[0, 1, 2]], np.int32))
+            w.add_object(SceneObject(f"o{i}", [Shape(mesh, Diffuse((1, 1, 1), albedo_texture=t))], Mat4()))
+        return w
+
+    a = ImageTexture.from_array(np.zeros((2, 2, 3)))
+    b = ImageTexture.from_array(np.ones((2, 2, 3)))
+    flat = flatten_scene(world_with([a, b, a, None]))
+    assert flat.textures == [a, b]
+    assert list(flat.triangles().texture_idx) == [0, 1, 0, -1]
+    with pytest.raises(ValueError):
+        flatten_scene(world_with([ImageTexture.from_array(np.zeros((1, 1, 3))) for _ in range(MAX_TEXTURES + 1)]))
+    px = ImageTexture.from_array(np.array([[[0.0, 0.5, 1.0]]])).load_pixels_u8()
+    assert px.dtype == np.uint8 and px.tolist() == [[[0, 128, 255]]]
+    assert ImageTexture.from_array(np.array([[[0, 128, 255]]])).load_pixels_u8().tolist() == [[[0, 128, 255]]]

@@ -25,6 +25,7 @@ import pytest
 
 from oracle import bvh_oracle
 from oracle import oracle as orc
+from tests import helpers
 
 
 # ------------------------------------------------------------------ Philox
@@ -380,3 +381,34 @@ def test_material_palette_first_appearance():
     rows = [(0, (1, 1, 1), 0, 1, 0, -1), (0, (0.5, 0.5, 0.5), 0, 1, 0, -1), (0, (1, 1, 1), 0, 1, 0, -1)]
     pal, idx = bvh_oracle.material_palette(rows)
     assert len(pal) == 2 and idx.tolist() == [0, 1, 0]
+
+
+@pytest.mark.parametrize("flip_v", [True, False])
+def test_oracle_texture_lookup_closed_form(flip_v):
+    """trace.py:96-136 in the oracle: on a floor with uv = world (x, z) and a
+    texture linear in its texel coordinates, the first-hit albedo is the
+    linear map at the hit's wrapped (and flipped) uv, for every camera ray."""
+    from roxy_dcc_amd.extract import flatten_scene
+
+    W, H = 16, 8
+    world, tex, _ = helpers.textured_floor_world(W, H, flip_v=flip_v)
+    flat = flatten_scene(world)
+    assert len(flat.textures) == 1
+    osc = helpers.flat_to_oracle(flat)
+    cam = world.active_camera
+    ocam = helpers.oracle_camera(cam, 48, 48)
+    acc = orc.Accum(48 * 48)
+    orc.render(osc, ocam, orc.params(1, 0, 0, 1, 10.0, (0.5, 0.5, 0.5), 3), 0, 1, acc)
+    rays = orc.camera_rays(ocam, 0)
+    o = np.asarray(cam.position, np.float64)
+    t = acc.depth.astype(np.float64)
+    hit = t > 0
+    assert hit.mean() > 0.3
+    p = o[None, :] + rays.astype(np.float64) * t[:, None]
+    want = helpers.textured_albedo_closed_form(p[hit], W, H, flip_v=flip_v)
+    got = acc.albedo[hit].astype(np.float64)
+    # away from the wrap seams (where uv, computed in fp32, may land on either side)
+    u, v = p[hit, 0] % 1.0, p[hit, 2] % 1.0
+    inner = (np.minimum(u, 1 - u) > 1e-3) & (np.minimum(v, 1 - v) > 1e-3)
+    assert inner.mean() > 0.9
+    assert np.max(np.abs(got[inner] - want[inner])) < 2e-4
