@@ -229,6 +229,17 @@ class IndexedMesh:
     def triangle_count(self):
         return len(self._tri_pos_idx)
 
+    @property
+    def vertex_count(self):
+        return len(self._positions)
+
+    @property
+    def groups(self):
+        return list(self._groups)
+
+    def group_for_triangle(self, i):
+        return self._groups[int(self._tri_group_idx[i])]
+
     def indexed_triangle_arrays(self, matrix=None, normal_matrix=None, dtype=np.float32):
         """World-space per-triangle arrays (scene/mesh.py:397-430): points
         through ``matrix`` with a perspective divide where |w| > 1e-12,
@@ -329,6 +340,10 @@ class Shape:
     def material_for_group(self, group_name):
         return self._group_materials.get(group_name, self._material)
 
+    @property
+    def material_groups(self):
+        return dict(self._group_materials)
+
 
 class SceneObject:
     def __init__(self, name="", shapes=None, transform=None, renderable=True):
@@ -342,6 +357,10 @@ class SceneObject:
     def add_child(self, child):
         child._parent = self
         self._children.append(child)
+
+    @property
+    def name(self):
+        return self._name
 
     @property
     def children(self):
