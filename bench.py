@@ -36,15 +36,18 @@ HBM_PEAK_GBS = 8000.0
 NODE_B, TRI_B, PRIM_B = 64, 36, 16
 
 
-def _load_traffic(name):
+def _load_traffic(name, samples_per_pass):
     """HBM bytes per extend launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by a copy kernel of known size), if recorded."""
+    WRITE_SIZE, scaled by a copy kernel of known size), if recorded for the
+    same workload and pass size (a launch's traffic grows with the pass)."""
     p = ROOT / "profiles" / f"traffic_{name}.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            t = json.loads(p.read_text())
         except Exception:
             return None
+        if t.get("samples_per_pass") == samples_per_pass:
+            return t.get("bytes_per_launch")
     return None
 
 
@@ -126,9 +129,13 @@ def _roofline(work, prof, steps, traffic):
     bytes_per_launch = alg / max(launches, 1)
     ms = prof.ms_extend / max(prof.n_extend, 1)
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    # measured HBM bytes over the same launch time: below `achieved` when
+    # node and triangle fetches hit in L2 / the Infinity Cache
+    hbm = traffic / (ms * 1e-3) / 1e9 if traffic else None
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_extend (closest-hit traversal)",
-            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "launches_per_step": launches}
+            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "launches_per_step": launches,
+            "hbm_traffic_gbs": hbm}
 
 
 def _stages(prof, steps):
@@ -183,7 +190,8 @@ def large_scene_extras(steps=2, extra_flags=0):
             "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
             "samples_per_s": W * H * st["samples"] * steps / dt,
             "stage_ms_per_render": _stages(integ.profile, steps),
-            "roofline": _roofline(work, integ.profile, steps, _load_traffic("large")),
+            "samples_per_pass": per_pass,
+            "roofline": _roofline(work, integ.profile, steps, _load_traffic("large", per_pass)),
             "traversal_work_per_render": work}
 
 
@@ -447,7 +455,7 @@ def main():
     rays = counts["rays"]
     ms_step = dt * 1e3 / args.steps
 
-    roofline = _roofline(work, prof, args.steps, _load_traffic(args.config))
+    roofline = _roofline(work, prof, args.steps, _load_traffic(args.config, per_pass))
 
     out = {
         "metric": "Mrays/s",

@@ -24,12 +24,18 @@ from .extract import extract_scene
 from .render_stats import RenderStats
 
 # Paths traced together per wavefront pass when samples_per_pass is not
-# given: enough to keep 256 CUs busy through the deeper bounces.
-TARGET_PATHS_PER_PASS = 1 << 21
+# given.  Every traversal launch ends with a tail in which the last deep
+# rays walk the BVH one dependent node fetch at a time (about 0.3-0.5 ms on
+# the scenes measured); a pass of 2^25 paths (about 200 bytes of workspace
+# each, 6.7 GB) spreads that tail over enough rays that it costs a few per
+# cent.  With a viewport attached, passes stay near 2^21 paths so the
+# display refreshes every few tens of milliseconds.
+TARGET_PATHS_PER_PASS = 1 << 25
+VIEWPORT_PATHS_PER_PASS = 1 << 21
 
 
-def default_samples_per_pass(width, height, samples):
-    per = max(1, TARGET_PATHS_PER_PASS // max(1, width * height))
+def default_samples_per_pass(width, height, samples, paths=TARGET_PATHS_PER_PASS):
+    per = max(1, paths // max(1, width * height))
     return int(max(1, min(samples, per)))
 
 
@@ -202,7 +208,8 @@ class TaichiRenderer:
             self._startup_progress.step("Extracting scene for GPU", "Flattening objects, materials and lights.")
         stats = extract_scene(world, device=self._device)
         self.scene = stats.scene
-        spp = self._samples_per_pass or default_samples_per_pass(W, H, self._samples)
+        spp = self._samples_per_pass or default_samples_per_pass(
+            W, H, self._samples, VIEWPORT_PATHS_PER_PASS if self._viewport else TARGET_PATHS_PER_PASS)
         if self._startup_progress:
             self._startup_progress.close()
             self._startup_progress = None

@@ -25,6 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .batch import Batch, RenderJob
+from .renderer import TARGET_PATHS_PER_PASS
 
 
 @dataclass
@@ -198,10 +199,11 @@ class RenderServer:
         return self.done
 
 
-def serial_loop(cache: SceneCache, requests, paths_per_pass=1 << 22, clock=time.perf_counter):
+def serial_loop(cache: SceneCache, requests, paths_per_pass=TARGET_PATHS_PER_PASS, clock=time.perf_counter):
     """One request at a time, in arrival order, each rendered to completion
-    in passes of up to `paths_per_pass` paths (1 = one sample per pass, the
-    reference renderer's launch-per-sample model)."""
+    in passes of up to `paths_per_pass` paths (the drop-in renderer's
+    default; 1 = one sample per pass, the reference renderer's
+    launch-per-sample model)."""
     import torch
 
     t0 = clock()

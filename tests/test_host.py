@@ -178,8 +178,11 @@ def test_renderer_validation_and_defaults():
 
 def test_default_samples_per_pass():
     assert default_samples_per_pass(256, 256, 16) == 16
-    assert default_samples_per_pass(1920, 1080, 16) == 1
-    assert default_samples_per_pass(512, 512, 64) == 8
+    assert default_samples_per_pass(1920, 1080, 16) == 16
+    assert default_samples_per_pass(1920, 1080, 64) == 16
+    assert default_samples_per_pass(512, 512, 256) == 128
+    assert default_samples_per_pass(1920, 1080, 16, paths=1 << 21) == 1
+    assert default_samples_per_pass(512, 512, 64, paths=1 << 21) == 8
 
 
 def test_render_stats_report():
