@@ -199,7 +199,7 @@ def _plain_render(R, torch, world, W, H, st, sc, seed, per_sample=False):
     """One request through the single-job path, then the frame read back.
     per_sample: the reference renderer's execution model, one launch
     sequence per sample (rendering/taichi_renderer.py:140-156); otherwise
-    the drop-in's policy of passes of up to 2M paths."""
+    the drop-in's policy of passes of up to 2^25 paths."""
     spp = 1 if per_sample else R.default_samples_per_pass(W, H, st["samples"])
     integ = R.Integrator(sc, W, H, spp)
     frame = R.DeviceFrame(W, H)
@@ -346,7 +346,7 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
         done = serial_loop(cache, requests(arr), paths_per_pass=1)
         row["serial_per_sample"] = latency_stats(done, time.perf_counter() - t0)
         same = 0
-        for policy in ("fair", "fifo"):
+        for policy in ("fair", "fifo", "srpt"):
             server = RenderServer(cache, max_jobs=32, policy=policy)
             t0 = time.perf_counter()
             done = server.run(requests(arr))
@@ -357,9 +357,9 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
     return {"workload": f"{n_requests} requests over {len(keys)} resident scenes (6 mesh, 2 large), "
                         "128^2..1920x1080, 4..64 spp or an error target (adaptive stop, threshold 0.005), depth 6, "
                         "Poisson arrivals; offered load relative to the "
-                        "serial loop's capacity; serial = passes of up to 4M paths, serial_per_sample = one "
+                        "serial loop's capacity; serial = the drop-in renderer (passes of up to 2^25 paths), serial_per_sample = one "
                         "sample per pass (the reference's launch-per-sample model); identical_images counts "
-                        "both server policies against the serial images",
+                        "the three server policies against the serial images",
             "scene_setup_s": setup, "serial_capacity_per_s": cap, "error_target_requests": len(adaptive_spp),
             "error_target_samples_taken": adaptive_spp, "sweep": rows}
 

@@ -338,6 +338,13 @@ def test_share_pass_water_filling():
     assert all(1 <= s <= need for s, (_, need) in zip(fifo, demands))
     assert fifo[0] == 4 and fifo[1] == 23 and fifo[2] == 1 and fifo[3] == 16
     assert sum(s * px for s, (px, _) in zip(fifo, demands)) <= 1 << 23
+    # arrival order against remaining work: the large frame first in line
+    late = [(1920 * 1080, 64), (512 * 512, 32)]
+    assert share_pass(late, 1 << 23, "fifo") == [3, 8]
+    assert share_pass(late, 1 << 23, "srpt") == [1, 24]
+    assert share_pass(demands, 1 << 23, "srpt") == [4, 23, 1, 16]
+    with pytest.raises(ValueError):
+        share_pass(demands, 1 << 23, "lifo")
 
 
 def _random_indexed_mesh(rng, n_pos=200, n_tris=500, normals=True, uvs=True):
