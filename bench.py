@@ -180,6 +180,26 @@ def large_scene_extras(steps=2, extra_flags=0):
     ev1.record()
     torch.cuda.synchronize()
     build_ms = ev0.elapsed_time(ev1) / reps
+    # core/bvh.py's object-median rule over the same resident triangles
+    # (fp64 vertices; host topology + per-level reduce and sorts)
+    om = GPUBVHBuilder()
+    om._reset_for_build(tb)
+    om._build_object_median_gpu()
+    v64 = [om.d[k].double().contiguous() for k in ("v0", "v1", "v2")]
+    om_ws = C.c_size_t()
+    nat.check(nat.lib().rx_bvh_object_median_workspace_size(n, om.OBJECT_MEDIAN_LEAF_SIZE, C.byref(om_ws)))
+    om_buf = torch.empty(om_ws.value, dtype=torch.uint8, device="cuda")
+    om_ref = om._ref_struct()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nat.check(nat.lib().rx_bvh_build_object_median(
+            v64[0].data_ptr(), v64[1].data_ptr(), v64[2].data_ptr(), n, om.OBJECT_MEDIAN_LEAF_SIZE,
+            om_buf.data_ptr(), om_ws.value, C.byref(om_ref), om.d["nodes"].data_ptr(), om.d["header"].data_ptr(),
+            om.d["order"].data_ptr(), nat.stream_ptr()))
+    om_ms = (time.perf_counter() - t0) * 1e3 / reps
+    om_shape = (om.node_count, om.depth)
+    del om, v64, om_buf
     integ, frame, work, dt, per_pass = _render_measure(R, nat, torch, world, W, H, st, sc, steps, nat.RX_FLAG_PROFILE,
                                                        extra_flags=extra_flags)
     counts = integ.counts()
@@ -187,6 +207,7 @@ def large_scene_extras(steps=2, extra_flags=0):
                         f"{sc.n_prims} primitives, {sc.n_lights} lights",
             "bvh_nodes": sc.builder.node_count, "bvh_depth": sc.builder.depth,
             "gpu_bvh_build_ms": build_ms, "scene_setup_s": setup,
+            "gpu_object_median_build_ms": om_ms, "object_median_nodes_depth": om_shape,
             "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
             "samples_per_s": W * H * st["samples"] * steps / dt,
             "stage_ms_per_render": _stages(integ.profile, steps),

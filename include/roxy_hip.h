@@ -311,6 +311,26 @@ int rx_bvh_build_morton(const float* d_v0, const float* d_v1, const float* d_v2,
 int rx_bvh_pack(const rx_bvh_ref* ref, int n_nodes, int depth,
                 rx_node* d_nodes, rx_bvh_header* d_header, void* stream);
 
+/* core/bvh.py:2-48 (BVHNode, the CPU tracer's per-mesh tree) built on the
+ * GPU and flattened in preorder into the same layouts.  Boxes are the union
+ * of Triangle.local_bounds (scene/mesh.py:68-76, fp64 vertex min/max -/+
+ * 1e-4), rounded to fp32 when stored; a node with more than max_leaf
+ * triangles at the root, or 4 below it (the recursion does not forward
+ * max_leaf_size, core/bvh.py:12-13), at depth <= 20 sorts them stably by
+ * the fp64 centroid ((v0 + v1) + v2) * (1 / 3) along the longest box axis
+ * (ties: x, then y) and splits at len // 2.  Vertices are n_tris*3 doubles
+ * (the reference's Python floats), finite.  The shape depends only on the
+ * count.  rx_bvh_build_object_median returns once the build has finished
+ * on `stream`. */
+int rx_bvh_object_median_shape(int n_tris, int max_leaf, int* n_nodes, int* depth);
+int rx_bvh_object_median_workspace_size(int n_tris, int max_leaf, size_t* bytes);
+int rx_bvh_build_object_median(const double* d_v0, const double* d_v1, const double* d_v2,
+                               int n_tris, int max_leaf,
+                               void* d_workspace, size_t workspace_bytes,
+                               const rx_bvh_ref* ref, rx_node* d_nodes,
+                               rx_bvh_header* d_header, int32_t* d_order,
+                               void* stream);
+
 /* Gather triangles into leaf order (bvh_builder.py:527-544) and precompute
  * edges.  d_n* may be NULL (face normal is used); d_mat_idx may be NULL
  * (material 0); d_has_uv may be NULL. */

@@ -114,6 +114,10 @@ SIGNATURES = {
     "rx_bvh_build_morton": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, C.c_size_t, C.POINTER(RxBvhRef), vp, vp,
                                       vp, vp]),
     "rx_bvh_pack": (C.c_int, [C.POINTER(RxBvhRef), C.c_int, C.c_int, vp, vp, vp]),
+    "rx_bvh_object_median_shape": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rx_bvh_object_median_workspace_size": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "rx_bvh_build_object_median": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp, C.c_size_t, C.POINTER(RxBvhRef),
+                                             vp, vp, vp, vp]),
     "rx_bvh_pack_triangles": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "rx_closest_hit": (C.c_int, [C.POINTER(RxScene), vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "rx_any_hit": (C.c_int, [C.POINTER(RxScene), vp, vp, C.c_int, vp, vp, vp]),

@@ -19,6 +19,12 @@ on the device and read back lazily for ``nodes``.
 build, numpy ``argpartition`` included, because the order numpy gives the
 elements inside each half is what defines that tree; its nodes are then
 packed on the device with ``rx_bvh_pack``.
+
+``method="object_median"`` (and ``build_object_median`` for fp64 vertex
+arrays) is core/bvh.py's BVHNode rule — fp64 boxes and centroids, a stable
+sort on the longest axis, a split at len // 2 — built on the GPU
+(``rx_bvh_build_object_median``) and flattened in preorder into the same
+layouts.
 """
 from __future__ import annotations
 
@@ -225,6 +231,45 @@ def morton_shape(n_tris: int, max_leaf: int) -> tuple[int, int]:
     return n.value, d.value
 
 
+# core/bvh.py:2: BVHNode's default leaf size (the only value below the root,
+# since the recursion does not forward the argument, :12-13)
+OBJECT_MEDIAN_LEAF_SIZE = 4
+
+
+def object_median_shape(n_tris: int, max_leaf: int = OBJECT_MEDIAN_LEAF_SIZE) -> tuple[int, int]:
+    """(node count, depth) of core/bvh.py's tree over n_tris triangles (the
+    split is always at len // 2, so the shape depends only on the count)."""
+    n, d = C.c_int(), C.c_int()
+    nat.check(nat.lib().rx_bvh_object_median_shape(int(n_tris), int(max_leaf), C.byref(n), C.byref(d)),
+              "bvh shape")
+    return n.value, d.value
+
+
+def build_object_median(v0, v1, v2, max_leaf_size=OBJECT_MEDIAN_LEAF_SIZE, device="cuda"):
+    """core/bvh.py's ``BVHNode(triangles, max_leaf_size=...)`` over one
+    mesh's triangles, built on the GPU and flattened in preorder.
+
+    v0, v1, v2: (n, 3) vertex arrays, taken as fp64 (the reference's
+    Python floats; scene/mesh.py:68-79 bounds and centroids are computed in
+    fp64).  Returns a ``GPUBVHBuilder`` holding the tree in the same
+    layouts as the other builds (``reference_arrays()``, ``ordered_tris``,
+    the packed nodes), so it can be traversed like any other."""
+    v = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, 3)) for a in (v0, v1, v2)]
+    n = len(v[0])
+    if n == 0:
+        raise ValueError("core/bvh.py builds over at least one triangle")
+    z = np.zeros((n, 3), np.float32)
+    batch = TriangleBatch(v[0].astype(np.float32), v[1].astype(np.float32), v[2].astype(np.float32), z, z, z,
+                          np.zeros(n, np.int32), z + 0.5, np.zeros(n, np.float32), np.ones(n, np.float32),
+                          np.zeros(n, np.float32))
+    b = GPUBVHBuilder(device=device)
+    b.OBJECT_MEDIAN_LEAF_SIZE = int(max_leaf_size)
+    b._reset_for_build(batch)
+    b._build_object_median_gpu(v64=v)
+    b._pack_triangles()
+    return b
+
+
 class GPUBVHBuilder:
     """Builds the BVH on the GPU and keeps it resident (bvh_builder.py:275).
 
@@ -236,6 +281,8 @@ class GPUBVHBuilder:
     MAX_DEPTH = 32
     BUILD_METHOD = "morton"
     MORTON_BITS = 10
+    # method="object_median": core/bvh.py's split rule, root leaf size
+    OBJECT_MEDIAN_LEAF_SIZE = OBJECT_MEDIAN_LEAF_SIZE
 
     def __init__(self, device="cuda"):
         self.device = device
@@ -262,6 +309,8 @@ class GPUBVHBuilder:
                 self._build_morton_gpu()
             elif method == "median":
                 self._build_median_host()
+            elif method == "object_median":
+                self._build_object_median_gpu()
             else:
                 raise ValueError(f"Unknown BVH build method: {method}")
             self._pack_triangles()
@@ -361,6 +410,33 @@ class GPUBVHBuilder:
             work.data_ptr(), ws.value, C.byref(ref), self.d["nodes"].data_ptr(), self.d["header"].data_ptr(),
             self.d["order"].data_ptr(), nat.stream_ptr()), "bvh build")
         self._workspace = work  # kept until the stream has consumed it
+
+    def _build_object_median_gpu(self, v64=None):
+        """core/bvh.py:2-48 as kernels (rx_bvh_build_object_median), over
+        the batch's vertices widened to fp64 (exact), or over ``v64`` (three
+        (n, 3) fp64 arrays) when the caller holds the fp64 originals."""
+        import torch
+
+        n = self._batch.triangle_count
+        leaf = int(self.OBJECT_MEDIAN_LEAF_SIZE)
+        self._n_nodes, self.depth = object_median_shape(n, leaf)
+        self._upload_geometry()
+        self._alloc_ref(self._n_nodes)
+        if v64 is None:
+            verts = [self.d[k].double().contiguous() for k in ("v0", "v1", "v2")]
+        else:
+            verts = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)).to(self.device)
+                     for a in v64]
+        ws = C.c_size_t()
+        nat.check(nat.lib().rx_bvh_object_median_workspace_size(n, leaf, C.byref(ws)), "bvh workspace")
+        work = torch.empty(ws.value, dtype=torch.uint8, device=self.device)
+        self.d["order"] = torch.empty(n, dtype=torch.int32, device=self.device)
+        ref = self._ref_struct()
+        # returns once the build has finished on the stream
+        nat.check(nat.lib().rx_bvh_build_object_median(
+            verts[0].data_ptr(), verts[1].data_ptr(), verts[2].data_ptr(), n, leaf, work.data_ptr(), ws.value,
+            C.byref(ref), self.d["nodes"].data_ptr(), self.d["header"].data_ptr(), self.d["order"].data_ptr(),
+            nat.stream_ptr()), "bvh build")
 
     def _build_median_host(self):
         """bvh_builder.py:488-525: host recursion with numpy argpartition,

@@ -81,8 +81,9 @@ def random_rays(flat, n, seed=7):
 
 
 def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0, spp_per_pass=None,
-                direct_light_mode=None, threads=None, world_tuple=None, flags=0):
-    """Render the same scene with the GPU library and the fp32 oracle."""
+                direct_light_mode=None, threads=None, world_tuple=None, flags=0, bvh_method=None):
+    """Render the same scene with the GPU library and the fp32 oracle
+    (both over the BVH `bvh_method` builds: Morton by default)."""
     import torch
 
     from roxy_dcc_amd import renderer as R
@@ -95,7 +96,7 @@ def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0
     mode = settings["direct_light_mode"] if direct_light_mode is None else direct_light_mode
     mode_id = R.TaichiRenderer._direct_light_mode_to_id(mode)
     dlmd = settings["direct_light_max_depth"]
-    ex = extract_scene(world)
+    ex = extract_scene(world, method=bvh_method)
     spp = spp_per_pass or samples
     integ = R.Integrator(ex.scene, W, H, spp)
     frame = R.DeviceFrame(W, H)
@@ -111,7 +112,10 @@ def render_pair(name, width=None, height=None, samples=4, max_depth=None, seed=0
     gpu = {k: getattr(frame, k).cpu().numpy() for k in ("color", "normal", "albedo", "depth", "display")}
     counts = integ.counts()
 
-    osc = flat_to_oracle(ex.flat)
+    if bvh_method == "object_median":
+        osc = flat_to_oracle(ex.flat, builder="object_median", max_leaf=4)
+    else:
+        osc = flat_to_oracle(ex.flat)
     ocam = oracle_camera(cam, W, H)
     oprm = orc.params(md, int(world._use_sky), mode_id, dlmd, 10.0, world._background_color, seed)
     acc = orc.Accum(W * H)

@@ -100,6 +100,75 @@ def test_median_build_matches_oracle():
     _assert_tree_equal(b, bvh_oracle.build_median(v0, v1, v2, 8))
 
 
+def _om_vertices(n, seed, kind):
+    rng = np.random.default_rng(seed)
+    if kind == "random":
+        v0 = rng.normal(size=(n, 3)) * 10.0
+        return v0, v0 + rng.normal(size=(n, 3)) * 0.3, v0 + rng.normal(size=(n, 3)) * 0.3
+    if kind == "grid":
+        # coarse grid and repeated triangles: equal centroid coordinates,
+        # whose current order a stable sort must keep
+        v0 = rng.integers(0, 6, size=(n, 3)).astype(np.float64)
+        return v0, v0 + [1.0, 0.0, 0.0], v0 + [0.0, 1.0, 0.0]
+    if kind == "flat":
+        # z = 0 plane with equal x and y extents: longest-axis ties (x wins)
+        v0 = np.stack([rng.integers(0, 16, n), rng.integers(0, 16, n), np.zeros(n)], 1).astype(np.float64)
+        return v0, v0 + [1.0, 0.0, 0.0], v0 + [0.0, 1.0, 0.0]
+    # degenerate triangles at the origin with either sign of zero: -0.0 and
+    # +0.0 compare equal in Python's sort, so their order must stay
+    v0 = np.where(rng.random((n, 3)) < 0.5, -0.0, 0.0)
+    v0[: n // 4] = rng.normal(size=(n // 4, 3))
+    return v0, v0.copy(), v0.copy()
+
+
+@pytest.mark.parametrize("n,leaf,kind", [(1, 4, "random"), (4, 4, "random"), (5, 4, "random"), (37, 16, "random"),
+                                         (3000, 4, "random"), (3000, 4, "grid"), (2000, 1, "flat"),
+                                         (500, 4, "zeros"), (60000, 4, "random")])
+def test_object_median_build_matches_oracle(n, leaf, kind):
+    """core/bvh.py's rule on the GPU: node arrays and leaf order equal the
+    oracle's fp64 restatement bit for bit."""
+    from roxy_dcc_amd.bvh import build_object_median
+
+    v0, v1, v2 = _om_vertices(n, 7 + n, kind)
+    print(f"object median n={n} root leaf={leaf} kind={kind}", flush=True)
+    b = build_object_median(v0, v1, v2, max_leaf_size=leaf)
+    _assert_tree_equal(b, bvh_oracle.build_object_median(v0, v1, v2, max_leaf=leaf))
+
+
+def test_object_median_build_matches_reference_golden():
+    """The GPU build against the reference's own BVHNode output
+    (tests/golden/object_median.npz): preorder ids, leaf ranges and order,
+    boxes rounded to fp32."""
+    from pathlib import Path
+
+    from roxy_dcc_amd.bvh import build_object_median
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "object_median.npz")
+    for k, (n, leaf, _seed) in enumerate(g["cases"]):
+        print(f"golden case {k}: n={n} root leaf={leaf}", flush=True)
+        b = build_object_median(g[f"case{k}_v0"], g[f"case{k}_v1"], g[f"case{k}_v2"], max_leaf_size=int(leaf))
+        r = b.reference_arrays()
+        assert b.node_count == len(g[f"case{k}_left"])
+        np.testing.assert_array_equal(b.ordered_tris, g[f"case{k}_order"])
+        for f in ("left", "right", "tri_start", "tri_count"):
+            np.testing.assert_array_equal(r[f], g[f"case{k}_{f}"], err_msg=f"{k} {f}")
+        for f in ("aabb_min", "aabb_max"):
+            np.testing.assert_array_equal(r[f], g[f"case{k}_{f}"].astype(np.float32), err_msg=f"{k} {f}")
+
+
+def test_object_median_scene_renders_like_the_oracle():
+    """A scene built with method="object_median" traverses and shades
+    exactly as the oracle over its own object-median tree."""
+    out = helpers.render_pair("mesh", 32, 32, samples=2, bvh_method="object_median")
+    b = out["extract"].scene.builder
+    tb = out["extract"].flat.triangles()
+    _assert_tree_equal(b, bvh_oracle.build_object_median(tb.v0, tb.v1, tb.v2, 4))
+    same = np.mean(np.all(out["gpu_color"] == out["cpu_color"], axis=1))
+    print(f"object-median mesh: exact {same:.4f} rays {out['gpu_rays']} {out['cpu_rays']}", flush=True)
+    assert out["gpu_rays"] == out["cpu_rays"]
+    assert same == 1.0
+
+
 def test_packed_nodes_match_reference_layout():
     """64-byte nodes hold the children's boxes and links of fields.py's layout."""
     rng = np.random.default_rng(5)

@@ -39,6 +39,45 @@ def test_morton_shape_closed_form_matches_recursion():
             assert (a.value, b.value) == bvh_oracle.closed_form_shape(n, leaf), (n, leaf)
 
 
+def test_object_median_shape_matches_the_oracle_tree():
+    """core/bvh.py's topology depends only on the count (the split is at
+    len // 2 whatever the geometry): the library's shape equals the
+    oracle's full build over real triangles for small counts and root leaf
+    sizes, and a count recursion of core/bvh.py:5-13 for large counts,
+    where the depth cap (depth > 20 makes a leaf) takes over past 4 * 2^21
+    triangles."""
+    from functools import lru_cache
+
+    from roxy_dcc_amd.bvh import object_median_shape
+
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 4, 5, 9, 17, 100, 1001):
+        v0 = rng.normal(size=(n, 3))
+        v1 = v0 + 0.1 * rng.normal(size=(n, 3))
+        v2 = v0 + 0.1 * rng.normal(size=(n, 3))
+        for leaf in (1, 4, 16):
+            t = bvh_oracle.build_object_median(v0, v1, v2, max_leaf=leaf)
+            assert object_median_shape(n, leaf) == (t.n_nodes, t.depth), (n, leaf)
+
+    @lru_cache(maxsize=None)
+    def rec(count, depth, root_leaf):
+        if count <= (root_leaf if depth == 0 else 4) or depth > 20:
+            return 1, depth
+        a = rec(count // 2, depth + 1, root_leaf)
+        b = rec(count - count // 2, depth + 1, root_leaf)
+        return 1 + a[0] + b[0], max(a[1], b[1])
+
+    for n in (69_630, 2_000_000, 8_388_608, 8_388_609, 20_000_000):
+        for leaf in (4, 100):
+            assert object_median_shape(n, leaf) == rec(n, 0, leaf), (n, leaf)
+    assert object_median_shape(20_000_000, 4) == (2 ** 22 - 1, 21)   # leaves of 9-10 at the cap
+    a, b = C.c_int(), C.c_int()
+    assert nat.lib().rx_bvh_object_median_shape(10, 0, C.byref(a), C.byref(b)) == nat.RX_ERR_INVALID
+    ws = C.c_size_t()
+    assert nat.lib().rx_bvh_object_median_workspace_size(nat.RX_MAX_TRIANGLES + 1, 4, C.byref(ws)) \
+        == nat.RX_ERR_CAPACITY
+
+
 def test_abi_errors_are_reported_without_touching_the_gpu():
     lib = nat.lib()
     a, b = C.c_int(), C.c_int()

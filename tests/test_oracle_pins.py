@@ -377,6 +377,26 @@ def test_object_median_rule_does_not_forward_leaf_size():
     assert t.tri_count[t.tri_count > 0].max() <= 4
 
 
+def test_object_median_oracle_matches_reference_bvhnode():
+    """The oracle's core/bvh.py restatement against the reference's own
+    BVHNode run on seeded meshes (tests/golden/object_median.npz, written by
+    tests/golden/make_object_median_golden.py): same preorder tree, leaf
+    order and fp64 boxes (stored rounded to fp32), including centroid ties
+    on a coarse grid, longest-axis ties on a flat mesh and -0.0 / +0.0
+    centroids."""
+    from pathlib import Path
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "object_median.npz")
+    for k, (n, leaf, _seed) in enumerate(g["cases"]):
+        t = bvh_oracle.build_object_median(g[f"case{k}_v0"], g[f"case{k}_v1"], g[f"case{k}_v2"], max_leaf=int(leaf))
+        assert t.n_nodes == len(g[f"case{k}_left"]), k
+        np.testing.assert_array_equal(t.order, g[f"case{k}_order"])
+        for f in ("left", "right", "tri_start", "tri_count"):
+            np.testing.assert_array_equal(getattr(t, f), g[f"case{k}_{f}"], err_msg=f"{k} {f}")
+        for f in ("aabb_min", "aabb_max"):
+            np.testing.assert_array_equal(getattr(t, f), g[f"case{k}_{f}"].astype(np.float32), err_msg=f"{k} {f}")
+
+
 def test_material_palette_first_appearance():
     rows = [(0, (1, 1, 1), 0, 1, 0, -1), (0, (0.5, 0.5, 0.5), 0, 1, 0, -1), (0, (1, 1, 1), 0, 1, 0, -1)]
     pal, idx = bvh_oracle.material_palette(rows)
