@@ -236,25 +236,30 @@ def _check_hits(g, o, label):
     assert g["counts"]["prim_tests"] == o["counts"]["prim_tests"]
 
 
-@pytest.mark.parametrize("name", ["cornell", "mesh"])
+@pytest.mark.parametrize("name", ["cornell", "mesh", "large"])
 def test_closest_hit_matches_oracle(name):
+    """Camera rays and 10^6 random rays (origins in the scene box,
+    directions uniform on the sphere) per configuration, the 2M-triangle
+    scene included."""
     from roxy_dcc_amd.extract import extract_scene
 
     world, w, h, _ = helpers.load(name)
     ex = extract_scene(world)
     osc = helpers.flat_to_oracle(ex.flat)
-    cam_rays = helpers.camera_rays_np(world.active_camera, 128, 128)
-    rnd = helpers.random_rays(ex.flat, 200_000)
+    cam_rays = helpers.camera_rays_np(world.active_camera, 480 if name == "large" else 128,
+                                      270 if name == "large" else 128)
+    rnd = helpers.random_rays(ex.flat, 1_000_000)
     for label, rays in (("camera", cam_rays), ("random", rnd)):
         _check_hits(_closest_gpu(ex.scene, rays), orc.closest_hit(osc, rays), f"{name}/{label}")
 
 
-def test_closest_hit_secondary_rays_match_oracle():
+@pytest.mark.parametrize("name", ["mesh", "large"])
+def test_closest_hit_secondary_rays_match_oracle(name):
     """Rays leaving hit points (offset 1e-4 along the oriented normal), as the
     bounce loop makes them (trace.py:234)."""
     from roxy_dcc_amd.extract import extract_scene
 
-    world, *_ = helpers.load("mesh")
+    world, *_ = helpers.load(name)
     ex = extract_scene(world)
     osc = helpers.flat_to_oracle(ex.flat)
     rays = helpers.camera_rays_np(world.active_camera, 256, 256)
@@ -268,7 +273,7 @@ def test_closest_hit_secondary_rays_match_oracle():
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     d = np.where((d * nrm[hit]).sum(1, keepdims=True) < 0, -d, d)
     sec = np.ascontiguousarray(np.concatenate([org.T, d.T]).astype(np.float32))
-    _check_hits(_closest_gpu(ex.scene, sec), orc.closest_hit(osc, sec), "mesh/secondary")
+    _check_hits(_closest_gpu(ex.scene, sec), orc.closest_hit(osc, sec), f"{name}/secondary")
 
 
 def test_any_hit_matches_oracle():
