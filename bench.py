@@ -140,7 +140,7 @@ def _roofline(work, prof, steps, traffic):
 
 def _stages(prof, steps):
     return {k: getattr(prof, "ms_" + k) / steps for k in
-            ("generate", "prim_scan", "extend", "shade", "connect", "accumulate")}
+            ("generate", "prim_scan", "reorder", "extend", "shade", "connect", "accumulate")}
 
 
 def large_scene_extras(steps=2, extra_flags=0):

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define RX_ABI_VERSION 2
+#define RX_ABI_VERSION 3
 
 /* status codes */
 #define RX_OK 0
@@ -203,6 +203,9 @@ typedef struct rx_camera {
 #define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
 #define RX_FLAG_PRIM_PREPASS 4   /* closest-primitive scan as its own pass over each ray
                                     queue (automatic above 256 analytic primitives) */
+#define RX_FLAG_KEEP_QUEUE_ORDER 32 /* rx_render: trace secondary ray queues in the order
+                                       shade wrote them instead of sorted by direction and
+                                       origin (same results, slower) */
 #define RX_FLAG_ADAPTIVE 16      /* adaptive stop: rendering/ray_tracer.py:531-541 (see
                                     adaptive_* below) */
 #define RX_FLAG_SMALL_BLOCKS 8   /* traversal in 64-thread blocks with the deep stack (the
@@ -270,6 +273,8 @@ typedef struct rx_profile {
     int32_t n_passes;
     float ms_prim_scan;     /* primitive pre-pass (scenes with many primitives) */
     int32_t n_prim_scan;
+    float ms_reorder;       /* secondary-ray reordering (keys, sort, gather) */
+    int32_t n_reorder;      /* its launches (three per reordered bounce) */
 } rx_profile;
 
 /* ---- library ---------------------------------------------------------- */

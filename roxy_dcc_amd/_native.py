@@ -32,6 +32,7 @@ RX_FLAG_COUNT_TRAFFIC = 2
 RX_FLAG_PRIM_PREPASS = 4
 RX_FLAG_SMALL_BLOCKS = 8
 RX_FLAG_ADAPTIVE = 16
+RX_FLAG_KEEP_QUEUE_ORDER = 32
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -91,6 +92,7 @@ class RxProfile(C.Structure):
         ("ms_connect", C.c_float), ("ms_accumulate", C.c_float),
         ("n_extend", i32), ("n_connect", i32), ("n_passes", i32),
         ("ms_prim_scan", C.c_float), ("n_prim_scan", i32),
+        ("ms_reorder", C.c_float), ("n_reorder", i32),
     ]
 
 

@@ -25,6 +25,8 @@
 
 #include "rx_launch.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace {
 
 using namespace rx;
@@ -81,9 +83,26 @@ struct ws_view {
     int err_stride;
     // traversal stack spill area (entries past the LDS part)
     int2* spill;
+    // reordered copy of a ray queue for k_extend (single-job passes): the
+    // sort keys and slot indices (double buffers), rocPRIM's scratch;
+    // hit_slot (set per launch) maps a position of the copy back to the
+    // queue slot its hit belongs to (null: the queue itself is traced)
+    gv4f* q_sorted;
+    uint32_t *sk_in, *sk_out, *si_in, *si_out;
+    void* sort_tmp;
+    size_t sort_bytes;
+    const uint32_t* hit_slot;
 };
 
-void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w) {
+// Radix-sort scratch for reordering n ray slots (rocPRIM's own query).
+size_t ray_sort_bytes(long long n) {
+    size_t sb = 0;
+    rocprim::double_buffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
+    if (rocprim::radix_sort_pairs(nullptr, sb, k, v, (unsigned int)(n > 0 ? n : 1), 0u, 30u) != hipSuccess) return 0;
+    return sb;
+}
+
+void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = false) {
     w.jobs = ar.take<rx_job>(n_jobs);
     w.spill = ar.take<int2>(trav_spill_entries(deep));
     w.path_off = ar.take<int32_t>(n_jobs + 1);
@@ -106,6 +125,20 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     float** pe[] = {&w.e_px, &w.e_py, &w.e_pz, &w.e_nx, &w.e_ny, &w.e_nz, &w.e_ax,
                     &w.e_ay, &w.e_az, &w.e_tx, &w.e_ty, &w.e_tz, &w.e_scale};
     for (float** f : pe) *f = ar.take<float>(P);
+    w.hit_slot = nullptr;
+    w.q_sorted = nullptr;
+    w.sk_in = w.sk_out = w.si_in = w.si_out = nullptr;
+    w.sort_tmp = nullptr;
+    w.sort_bytes = 0;
+    if (sort) {
+        w.q_sorted = ar.take<gv4f>(2 * P);
+        w.sk_in = ar.take<uint32_t>(P);
+        w.sk_out = ar.take<uint32_t>(P);
+        w.si_in = ar.take<uint32_t>(P);
+        w.si_out = ar.take<uint32_t>(P);
+        w.sort_bytes = ray_sort_bytes(P);
+        w.sort_tmp = ar.take_bytes(w.sort_bytes);
+    }
     w.n_jobs = n_jobs;
     w.P = P;
 }
@@ -576,7 +609,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 // primitive scan first (intersect.py:259-287), or its
                 // result from k_prim_scan
                 if (PRE) {
-                    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[i];
+                    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[w.hit_slot ? (int)w.hit_slot[i] : i];
                     prim_t = h.x;
                     prim_id = __float_as_int(h.y);
                 } else {
@@ -648,7 +681,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     u = best_u;
                     v = best_v;
                 }
-                put_hit(w, i, t, kind, id, u, v);
+                put_hit(w, w.hit_slot ? (int)w.hit_slot[i] : i, t, kind, id, u, v);
                 w.p_rays[p] += 1;
                 i = -1;
             }
@@ -1282,8 +1315,8 @@ struct stage_timer {
     int finish(rx_profile* prof) {
         if (!on) return RX_OK;
         RX_TRY_HIP(hipStreamSynchronize(s));
-        float sum[6] = {0, 0, 0, 0, 0, 0};
-        int cnt[6] = {0, 0, 0, 0, 0, 0};
+        float sum[7] = {0, 0, 0, 0, 0, 0, 0};
+        int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int k = 0; k < n; k += 2) {
             float ms = 0.0f;
             RX_TRY_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
@@ -1303,6 +1336,8 @@ struct stage_timer {
             prof->n_connect += cnt[3];
             prof->ms_prim_scan += sum[5];
             prof->n_prim_scan += cnt[5];
+            prof->ms_reorder += sum[6];
+            prof->n_reorder += cnt[6];
         }
         return RX_OK;
     }
@@ -1331,6 +1366,7 @@ struct pass_plan {
     int trav_per_cu;         // resident traversal blocks per CU (0 = default)
     int max_spp;             // most samples per pass of any job
     bool adaptive;           // some job has RX_FLAG_ADAPTIVE
+    bool sort_rays;          // reorder secondary ray queues for k_extend (rx_render only)
 };
 
 // Scenes with more analytic primitives than this get their closest-
@@ -1342,26 +1378,102 @@ struct pass_plan {
 // RX_FLAG_PRIM_PREPASS asks for the pre-pass at any count.
 constexpr int PRIM_INLINE_MAX = 256;
 
+// ---------------------------------------------------------------- reorder
+// Secondary rays leave in scattered directions, so lanes of one wave walk
+// unrelated subtrees.  Before k_extend of a bounce > 0 the queue is sorted
+// by a key that puts rays with similar directions and nearby origins next
+// to each other: the direction quantised to 8 levels per axis (9 bits,
+// most significant) over a 21-bit Morton code of the origin in the scene
+// box (7 bits per axis).  k_extend traces the sorted copy and writes each
+// hit back to its ray's own queue slot, so shading, path state and every
+// result are those of the unsorted queue.
+__device__ __forceinline__ uint32_t spread7(uint32_t v) {
+    v &= 0x7fu;
+    v = (v | (v << 8)) & 0x0000f00fu;
+    v = (v | (v << 4)) & 0x000c30c3u;
+    v = (v | (v << 2)) & 0x00249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t ray_key(f3 o, f3 d, const rx_bvh_header& h) {
+    uint32_t q[3];
+    const float oc[3] = {o.x, o.y, o.z};
+    for (int a = 0; a < 3; ++a) {
+        float u = (oc[a] - h.bmin[a]) / fmaxf(h.bmax[a] - h.bmin[a], 1e-20f);
+        q[a] = (uint32_t)(fminf(fmaxf(u, 0.0f), 1.0f) * 127.0f);
+    }
+    uint32_t dx = (uint32_t)fminf((d.x + 1.0f) * 4.0f, 7.0f);
+    uint32_t dy = (uint32_t)fminf((d.y + 1.0f) * 4.0f, 7.0f);
+    uint32_t dz = (uint32_t)fminf((d.z + 1.0f) * 4.0f, 7.0f);
+    uint32_t dq = (dx << 6) | (dy << 3) | dz;
+    return (dq << 21) | (spread7(q[0]) << 2) | (spread7(q[1]) << 1) | spread7(q[2]);
+}
+
+__global__ void k_ray_keys(ws_view w, int b, int n, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    const rx_bvh_header h = gload(w.jobs[0].scene.header);
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        f3 o, d;
+        (void)get_ray(w, b, i, o, d);
+        key[i] = ray_key(o, d, h);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_ray_gather(ws_view w, int b, int n, const uint32_t* __restrict__ idx, gv4f* __restrict__ out) {
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)idx[i];
+        RX_GLOBAL gv4f* o = (RX_GLOBAL gv4f*)out + 2 * (long long)i;
+        o[0] = q[0];
+        o[1] = q[1];
+    }
+}
+
+// Queues shorter than this are traced as they are.
+constexpr int SORT_MIN_RAYS = 8192;
+
 template <int SBS>
-int enqueue_pass(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+    ws_view w = w0;
     int gp = grid_for(w.P, BS, 16);
     int gt = trav_grid(w.P, SBS == BS_DEEP, pl.trav_per_cu);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
+        ws_view we = w;
+        if (d > 0 && pl.sort_rays && w.q_sorted && w.sort_bytes > 0) {
+            // the queue length is on the device: one synchronisation per
+            // bounce (this path is never captured in a graph)
+            int cnt = 0;
+            RX_TRY_HIP(hipMemcpyAsync(&cnt, w.counts + qi, sizeof(int), hipMemcpyDeviceToHost, s));
+            RX_TRY_HIP(hipStreamSynchronize(s));
+            if (cnt >= SORT_MIN_RAYS) {
+                int gk = grid_for(cnt, BS, 16);
+                RX_STAGE(tm, 6, (k_ray_keys<<<gk, BS, 0, s>>>(w, qi, cnt, w.sk_in, w.si_in)));
+                rocprim::double_buffer<uint32_t> keys(w.sk_in, w.sk_out), slots(w.si_in, w.si_out);
+                size_t sb = w.sort_bytes;
+                int rs = tm.begin(6);
+                if (rs) return rs;
+                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)cnt, 0u, 30u, s));
+                rs = tm.end();
+                if (rs) return rs;
+                RX_STAGE(tm, 6, (k_ray_gather<<<gk, BS, 0, s>>>(w, qi, cnt, slots.current(), w.q_sorted)));
+                we.q_ray[qi] = w.q_sorted;
+                we.hit_slot = slots.current();
+            }
+        }
         if (pl.prim_prepass) {
             if (pl.count) {
                 RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(w, qi)));
+                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(we, qi)));
             } else {
                 RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(w, qi)));
+                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(we, qi)));
             }
         } else if (pl.count) {
-            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(w, qi)));
+            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(we, qi)));
         } else {
-            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(w, qi)));
+            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(we, qi)));
         }
         RX_STAGE(tm, 2, (k_shade<<<gp, BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
@@ -1426,7 +1538,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false, false, 0, 0, false};
+    pass_plan plan{0, 0, false, false, false, 0, 0, false, false};
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -1457,21 +1569,22 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
     return RX_OK;
 }
 
-size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride) {
+// sort: room for reordering secondary rays (the single-job entry point)
+size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride, bool sort = false) {
     rx_arena ar{nullptr, 0, 0};
     ws_view w;
-    carve(ar, n_jobs, P, deep, err_stride, w);
+    carve(ar, n_jobs, P, deep, err_stride, w, sort);
     return ar.off + 256;
 }
 
 int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, bool deep, int err_stride,
-               ws_view& w) {
-    size_t need = workspace_bytes_for(n_jobs, P, deep, err_stride);
+               ws_view& w, bool sort = false) {
+    size_t need = workspace_bytes_for(n_jobs, P, deep, err_stride, sort);
     RX_REQUIRE(d_workspace && bytes >= need, RX_ERR_WORKSPACE, "render workspace too small: need %zu bytes, have %zu",
                need, bytes);
     rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, 0};
     ar.off = (256 - (reinterpret_cast<uintptr_t>(d_workspace) & 255)) & 255;
-    carve(ar, n_jobs, P, deep, err_stride, w);
+    carve(ar, n_jobs, P, deep, err_stride, w, sort);
     w.NP = NP;
     return RX_OK;
 }
@@ -1496,7 +1609,7 @@ extern "C" int rx_render_workspace_size(int width, int height, int samples_per_p
                "width, height and samples_per_pass must be positive");
     long long P = (long long)width * height * samples_per_pass;
     RX_REQUIRE(P <= 0x7fffffffLL, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
-    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth), samples_per_pass);
+    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth), samples_per_pass, true);
     return RX_OK;
 }
 
@@ -1523,7 +1636,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     j.pixel_offset = 0;
     if (n_samples == 0) return RX_OK;
     ws_view w;
-    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, pl.max_spp, w);
+    st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, pl.max_spp, w, true);
     if (st) return st;
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
@@ -1533,6 +1646,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
     tm.s = s;
+    pl.sort_rays = !(params->flags & RX_FLAG_KEEP_QUEUE_ORDER);
     int spp = params->samples_per_pass;
     for (int done = 0; done < n_samples; done += spp) {
         st = enqueue(w, pl, d_counters, tm, s);

@@ -320,6 +320,39 @@ def test_render_independent_of_pass_split():
     np.testing.assert_array_equal(a["gpu_depth"], b["gpu_depth"])
 
 
+def _render_mesh(flags, W=128, H=128, spp=4):
+    import torch
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, _, _, st = helpers.load("mesh")
+    ex = extract_scene(world)
+    integ = R.Integrator(ex.scene, W, H, spp)
+    frame = R.DeviceFrame(W, H)
+    prm = R.params_struct(st["max_depth"], int(world._use_sky), 0, st["direct_light_max_depth"], 10.0,
+                          world._background_color, 3, spp, flags | nat.RX_FLAG_PROFILE | nat.RX_FLAG_COUNT_TRAFFIC)
+    integ.render(frame, R.camera_struct(world.active_camera, W, H), prm, 0, spp)
+    torch.cuda.synchronize()
+    return frame.color.cpu().numpy(), integ.counts(), integ.profile
+
+
+def test_reordered_secondary_queues_change_nothing():
+    """k_extend traces each secondary queue sorted by direction and origin
+    and writes every hit back to its ray's slot: the image and every
+    traversal counter equal those of the queue traced in shade's order."""
+    from roxy_dcc_amd import _native as nat
+
+    sorted_img, sorted_cnt, sorted_prof = _render_mesh(0)
+    plain_img, plain_cnt, plain_prof = _render_mesh(nat.RX_FLAG_KEEP_QUEUE_ORDER)
+    print(f"reorder launches {sorted_prof.n_reorder} ({sorted_prof.ms_reorder:.2f} ms); extend "
+          f"{sorted_prof.ms_extend:.2f} vs {plain_prof.ms_extend:.2f} ms", flush=True)
+    assert sorted_prof.n_reorder >= 3 and plain_prof.n_reorder == 0
+    np.testing.assert_array_equal(sorted_img, plain_img)
+    assert sorted_cnt == plain_cnt
+
+
 def test_render_all_lights_mode_matches_oracle():
     from roxy_dcc_amd import scenes
 
