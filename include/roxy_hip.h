@@ -203,9 +203,11 @@ typedef struct rx_camera {
 #define RX_FLAG_COUNT_TRAFFIC 2  /* count node pops / triangle / primitive tests */
 #define RX_FLAG_PRIM_PREPASS 4   /* closest-primitive scan as its own pass over each ray
                                     queue (automatic above 256 analytic primitives) */
-#define RX_FLAG_KEEP_QUEUE_ORDER 32 /* rx_render: trace secondary ray queues in the order
-                                       shade wrote them instead of sorted by direction and
-                                       origin (same results, slower) */
+#define RX_FLAG_KEEP_QUEUE_ORDER 32 /* rx_render: trace every ray queue in path order
+                                       (primary rays sample-major, secondary rays as shade
+                                       wrote them) instead of pixel-major primaries and
+                                       secondaries sorted by direction and origin (same
+                                       results, slower) */
 #define RX_FLAG_ADAPTIVE 16      /* adaptive stop: rendering/ray_tracer.py:531-541 (see
                                     adaptive_* below) */
 #define RX_FLAG_SMALL_BLOCKS 8   /* traversal in 64-thread blocks with the deep stack (the

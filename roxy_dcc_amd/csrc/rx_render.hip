@@ -453,6 +453,30 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     }
 }
 
+// The primary rays of a single-job pass once more, pixel-major: position
+// k * ns + s holds sample s of pixel slot k (ns samples this pass), so a
+// wave traces the samples of one or a few pixels — nearly identical rays —
+// instead of 64 neighbouring pixels of one sample.  Same camera_dir inputs
+// as k_generate, so the same rays; slot[] maps each position back to the
+// ray's queue slot for its hit.
+__global__ __launch_bounds__(BS) void k_generate_pm(ws_view w, gv4f* __restrict__ out, uint32_t* __restrict__ slot) {
+    const rx_job& J = w.jobs[0];
+    const int npix = J.camera.width * J.camera.height;
+    const int ns = pass_samples(J);
+    const long long n = (long long)npix * ns;
+    for (long long ip = (long long)blockIdx.x * BS + threadIdx.x; ip < n; ip += (long long)gridDim.x * BS) {
+        int k = (int)(ip / ns), s = (int)(ip % ns);
+        long long local = (long long)s * npix + k;
+        int pix = slot_pixel(J.camera, k);
+        f3 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
+        f3 org = ld3(J.camera.pos);
+        RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)out + 2 * ip;
+        q[0] = gv4f{org.x, org.y, org.z, __int_as_float((int)(w.path_off[0] + local))};
+        q[1] = gv4f{rd.x, rd.y, rd.z, 0.0f};
+        slot[ip] = (uint32_t)(w.q_off[0] + local);
+    }
+}
+
 // Closest analytic primitive along a ray: the linear scan of
 // rendering/taichi/intersect.py:259-287 (ascending index, strict
 // 0.001 < t < closest, so the lowest index wins a tie).
@@ -1437,10 +1461,18 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     int gp = grid_for(w.P, BS, 16);
     int gt = trav_grid(w.P, SBS == BS_DEEP, pl.trav_per_cu);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
+    // single-job passes of several samples trace their primary rays
+    // pixel-major (k_generate_pm); later bounces are sorted below
+    const bool pm = pl.sort_rays && w.q_sorted && w.n_jobs == 1 && pl.max_spp > 1;
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
+    if (pm) RX_STAGE(tm, 6, (k_generate_pm<<<gp, BS, 0, s>>>(w, w.q_sorted, w.si_in)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
         ws_view we = w;
+        if (d == 0 && pm) {
+            we.q_ray[0] = w.q_sorted;
+            we.hit_slot = w.si_in;
+        }
         if (d > 0 && pl.sort_rays && w.q_sorted && w.sort_bytes > 0) {
             // the queue length is on the device: one synchronisation per
             // bounce (this path is never captured in a graph)
