@@ -35,7 +35,7 @@ from . import _native as nat
 # for a batch workspace (path state is about 200 bytes per slot).
 MAX_PATHS_PER_PASS = (1 << 31) - 1
 WORKSPACE_BUDGET = 24 << 30
-BYTES_PER_PATH = 200
+BYTES_PER_PATH = 250
 
 
 @dataclass(frozen=True)

@@ -102,7 +102,7 @@ size_t ray_sort_bytes(long long n) {
     return sb;
 }
 
-void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = false) {
+void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
     w.jobs = ar.take<rx_job>(n_jobs);
     w.spill = ar.take<int2>(trav_spill_entries(deep));
     w.path_off = ar.take<int32_t>(n_jobs + 1);
@@ -453,27 +453,30 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     }
 }
 
-// The primary rays of a single-job pass once more, pixel-major: position
-// k * ns + s holds sample s of pixel slot k (ns samples this pass), so a
-// wave traces the samples of one or a few pixels — nearly identical rays —
-// instead of 64 neighbouring pixels of one sample.  Same camera_dir inputs
-// as k_generate, so the same rays; slot[] maps each position back to the
-// ray's queue slot for its hit.
+// The primary rays once more, pixel-major within each job: position
+// q_off[j] + k * ns + s of the queue holds sample s of pixel slot k (ns
+// samples of the job this pass), so a wave traces the samples of one or a
+// few pixels — nearly identical rays — instead of 64 neighbouring pixels
+// of one sample.  Same camera_dir inputs as k_generate, so the same rays;
+// slot[] maps each position back to the ray's queue slot for its hit.
 __global__ __launch_bounds__(BS) void k_generate_pm(ws_view w, gv4f* __restrict__ out, uint32_t* __restrict__ slot) {
-    const rx_job& J = w.jobs[0];
-    const int npix = J.camera.width * J.camera.height;
-    const int ns = pass_samples(J);
-    const long long n = (long long)npix * ns;
-    for (long long ip = (long long)blockIdx.x * BS + threadIdx.x; ip < n; ip += (long long)gridDim.x * BS) {
-        int k = (int)(ip / ns), s = (int)(ip % ns);
-        long long local = (long long)s * npix + k;
-        int pix = slot_pixel(J.camera, k);
+    const long long total = w.q_off[w.n_jobs];
+    const int stride = gridDim.x * BS;
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < total; i += stride) {
+        const int jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
+        const rx_job& J = w.jobs[jx];
+        const int npix = J.camera.width * J.camera.height;
+        const int ns = pass_samples(J);
+        const long long li = i - w.q_off[jx];
+        const int k = (int)(li / ns), s = (int)(li % ns);
+        const long long local = (long long)s * npix + k;
+        const int pix = slot_pixel(J.camera, k);
         f3 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
         f3 org = ld3(J.camera.pos);
-        RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)out + 2 * ip;
-        q[0] = gv4f{org.x, org.y, org.z, __int_as_float((int)(w.path_off[0] + local))};
+        RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)out + 2 * i;
+        q[0] = gv4f{org.x, org.y, org.z, __int_as_float((int)(w.path_off[jx] + local))};
         q[1] = gv4f{rd.x, rd.y, rd.z, 0.0f};
-        slot[ip] = (uint32_t)(w.q_off[0] + local);
+        slot[i] = (uint32_t)(w.q_off[jx] + local);
     }
 }
 
@@ -1390,7 +1393,7 @@ struct pass_plan {
     int trav_per_cu;         // resident traversal blocks per CU (0 = default)
     int max_spp;             // most samples per pass of any job
     bool adaptive;           // some job has RX_FLAG_ADAPTIVE
-    bool sort_rays;          // reorder secondary ray queues for k_extend (rx_render only)
+    bool sort_rays;          // reorder ray queues for k_extend (no job asks to keep the order)
 };
 
 // Scenes with more analytic primitives than this get their closest-
@@ -1405,12 +1408,13 @@ constexpr int PRIM_INLINE_MAX = 256;
 // ---------------------------------------------------------------- reorder
 // Secondary rays leave in scattered directions, so lanes of one wave walk
 // unrelated subtrees.  Before k_extend of a bounce > 0 the queue is sorted
-// by a key that puts rays with similar directions and nearby origins next
-// to each other: the direction quantised to 8 levels per axis (9 bits,
-// most significant) over a 21-bit Morton code of the origin in the scene
-// box (7 bits per axis).  k_extend traces the sorted copy and writes each
-// hit back to its ray's own queue slot, so shading, path state and every
-// result are those of the unsorted queue.
+// by a 30-bit key that puts rays of one job with similar directions and
+// nearby origins next to each other: the job (most significant, as many
+// bits as the batch needs), the direction quantised to 8 levels per axis
+// (9 bits), and a Morton code of the origin in the job's scene box with
+// the bits left (7 per axis for a single job).  k_extend traces the sorted
+// copy and writes each hit back to its ray's own queue slot, so shading,
+// path state and every result are those of the unsorted queue.
 __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     v &= 0x7fu;
     v = (v | (v << 8)) & 0x0000f00fu;
@@ -1419,51 +1423,86 @@ __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint32_t ray_key(f3 o, f3 d, const rx_bvh_header& h) {
+// ob: origin bits per axis (0..7)
+__device__ __forceinline__ uint32_t ray_key(f3 o, f3 d, const rx_bvh_header& h, uint32_t job, int ob) {
     uint32_t q[3];
     const float oc[3] = {o.x, o.y, o.z};
     for (int a = 0; a < 3; ++a) {
         float u = (oc[a] - h.bmin[a]) / fmaxf(h.bmax[a] - h.bmin[a], 1e-20f);
-        q[a] = (uint32_t)(fminf(fmaxf(u, 0.0f), 1.0f) * 127.0f);
+        q[a] = (uint32_t)(fminf(fmaxf(u, 0.0f), 1.0f) * 127.0f) >> (7 - ob);
     }
     uint32_t dx = (uint32_t)fminf((d.x + 1.0f) * 4.0f, 7.0f);
     uint32_t dy = (uint32_t)fminf((d.y + 1.0f) * 4.0f, 7.0f);
     uint32_t dz = (uint32_t)fminf((d.z + 1.0f) * 4.0f, 7.0f);
     uint32_t dq = (dx << 6) | (dy << 3) | dz;
-    return (dq << 21) | (spread7(q[0]) << 2) | (spread7(q[1]) << 1) | spread7(q[2]);
+    uint32_t mo = (spread7(q[0]) << 2) | (spread7(q[1]) << 1) | spread7(q[2]);
+    return (job << (9 + 3 * ob)) | (dq << (3 * ob)) | mo;
 }
 
-__global__ void k_ray_keys(ws_view w, int b, int n, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
-    const rx_bvh_header h = gload(w.jobs[0].scene.header);
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        f3 o, d;
-        (void)get_ray(w, b, i, o, d);
-        key[i] = ray_key(o, d, h);
+// Keys of the first n slots of queue b; slots past the queue's length
+// (read on the device) get the largest key, so a sort of the whole
+// capacity leaves them last.
+__global__ void k_ray_keys(ws_view w, int b, long long n, int ob, uint32_t* __restrict__ key,
+                           uint32_t* __restrict__ idx) {
+    const int count = w.counts[b];
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
+        uint32_t k = 0xffffffffu;
+        if (i < count) {
+            f3 o, d;
+            const int p = get_ray(w, b, (int)i, o, d);
+            const int jx = job_of_path(w, p);
+            const rx_scene& sc = w.jobs[jx].scene;
+            if (sc.n_tris > 0) {
+                k = ray_key(o, d, gload(sc.header), (uint32_t)jx, ob);
+            } else {
+                k = (uint32_t)jx << (9 + 3 * ob);
+            }
+        }
+        key[i] = k;
         idx[i] = (uint32_t)i;
     }
 }
 
-__global__ void k_ray_gather(ws_view w, int b, int n, const uint32_t* __restrict__ idx, gv4f* __restrict__ out) {
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+__global__ void k_ray_gather(ws_view w, int b, long long n, const uint32_t* __restrict__ idx,
+                             gv4f* __restrict__ out) {
+    const long long m = n < (long long)w.counts[b] ? n : (long long)w.counts[b];
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < m; i += (long long)gridDim.x * BS) {
         const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)idx[i];
-        RX_GLOBAL gv4f* o = (RX_GLOBAL gv4f*)out + 2 * (long long)i;
+        RX_GLOBAL gv4f* o = (RX_GLOBAL gv4f*)out + 2 * i;
         o[0] = q[0];
         o[1] = q[1];
     }
 }
 
-// Queues shorter than this are traced as they are.
+// Queues shorter than this are traced as they are (when their length is
+// known on the host), and passes of fewer path slots than SORT_MIN_PATHS
+// do not read lengths back at all: for them the synchronisation per
+// bounce costs more than a sort saves (a loop of one-sample passes of
+// small frames would pay it thousands of times).
 constexpr int SORT_MIN_RAYS = 8192;
+constexpr long long SORT_MIN_PATHS = 1 << 20;
 
+// Origin bits per axis of the sort key for a batch of n_jobs.
+inline int key_origin_bits(int n_jobs) {
+    int jb = 0;
+    while ((1 << jb) < n_jobs) ++jb;
+    int ob = (30 - 9 - jb) / 3;
+    return ob > 7 ? 7 : (ob < 0 ? 0 : ob);
+}
+
+// host_counts: the caller may synchronise (not under graph capture), so
+// each queue's length is read back and only that many rays are sorted;
+// otherwise the whole capacity is sorted, idle slots last.
 template <int SBS>
-int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
+                 bool host_counts) {
     ws_view w = w0;
     int gp = grid_for(w.P, BS, 16);
     int gt = trav_grid(w.P, SBS == BS_DEEP, pl.trav_per_cu);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
-    // single-job passes of several samples trace their primary rays
+    // passes of several samples per pixel trace their primary rays
     // pixel-major (k_generate_pm); later bounces are sorted below
-    const bool pm = pl.sort_rays && w.q_sorted && w.n_jobs == 1 && pl.max_spp > 1;
+    const bool pm = pl.sort_rays && w.q_sorted && pl.max_spp > 1;
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     if (pm) RX_STAGE(tm, 6, (k_generate_pm<<<gp, BS, 0, s>>>(w, w.q_sorted, w.si_in)));
     for (int d = 0; d < pl.max_depth; ++d) {
@@ -1474,22 +1513,26 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
             we.hit_slot = w.si_in;
         }
         if (d > 0 && pl.sort_rays && w.q_sorted && w.sort_bytes > 0) {
-            // the queue length is on the device: one synchronisation per
-            // bounce (this path is never captured in a graph)
-            int cnt = 0;
-            RX_TRY_HIP(hipMemcpyAsync(&cnt, w.counts + qi, sizeof(int), hipMemcpyDeviceToHost, s));
-            RX_TRY_HIP(hipStreamSynchronize(s));
-            if (cnt >= SORT_MIN_RAYS) {
-                int gk = grid_for(cnt, BS, 16);
-                RX_STAGE(tm, 6, (k_ray_keys<<<gk, BS, 0, s>>>(w, qi, cnt, w.sk_in, w.si_in)));
+            long long n = w.P;
+            if (host_counts && w.P < SORT_MIN_PATHS) {
+                n = 0;
+            } else if (host_counts) {
+                int cnt = 0;
+                RX_TRY_HIP(hipMemcpyAsync(&cnt, w.counts + qi, sizeof(int), hipMemcpyDeviceToHost, s));
+                RX_TRY_HIP(hipStreamSynchronize(s));
+                n = cnt >= SORT_MIN_RAYS ? cnt : 0;
+            }
+            if (n > 0) {
+                int gk = grid_for(n, BS, 16);
+                RX_STAGE(tm, 6, (k_ray_keys<<<gk, BS, 0, s>>>(w, qi, n, key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
                 rocprim::double_buffer<uint32_t> keys(w.sk_in, w.sk_out), slots(w.si_in, w.si_out);
                 size_t sb = w.sort_bytes;
                 int rs = tm.begin(6);
                 if (rs) return rs;
-                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)cnt, 0u, 30u, s));
+                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)n, 0u, 30u, s));
                 rs = tm.end();
                 if (rs) return rs;
-                RX_STAGE(tm, 6, (k_ray_gather<<<gk, BS, 0, s>>>(w, qi, cnt, slots.current(), w.q_sorted)));
+                RX_STAGE(tm, 6, (k_ray_gather<<<gk, BS, 0, s>>>(w, qi, n, slots.current(), w.q_sorted)));
                 we.q_ray[qi] = w.q_sorted;
                 we.hit_slot = slots.current();
             }
@@ -1522,9 +1565,10 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     return RX_OK;
 }
 
-int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
-    if (pl.deep) return enqueue_pass<BS_DEEP>(w, pl, ctr, tm, s);
-    return enqueue_pass<BS_SHALLOW>(w, pl, ctr, tm, s);
+int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
+            bool host_counts) {
+    if (pl.deep) return enqueue_pass<BS_DEEP>(w, pl, ctr, tm, s, host_counts);
+    return enqueue_pass<BS_SHALLOW>(w, pl, ctr, tm, s, host_counts);
 }
 
 int check_job(const rx_job& j, int idx) {
@@ -1570,7 +1614,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false, false, 0, 0, false, false};
+    pass_plan plan{0, 0, false, false, false, 0, 0, false, true};
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -1589,6 +1633,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
         if (j.params.samples_per_pass > plan.max_spp) plan.max_spp = j.params.samples_per_pass;
         if (j.params.flags & RX_FLAG_ADAPTIVE) plan.adaptive = true;
         if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
+        if (j.params.flags & RX_FLAG_KEEP_QUEUE_ORDER) plan.sort_rays = false;
         if (j.scene.n_prims > 0 && (j.scene.n_prims > PRIM_INLINE_MAX || (j.params.flags & RX_FLAG_PRIM_PREPASS)))
             plan.prim_prepass = true;
     }
@@ -1602,7 +1647,7 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
 }
 
 // sort: room for reordering secondary rays (the single-job entry point)
-size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride, bool sort = false) {
+size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride, bool sort = true) {
     rx_arena ar{nullptr, 0, 0};
     ws_view w;
     carve(ar, n_jobs, P, deep, err_stride, w, sort);
@@ -1610,7 +1655,7 @@ size_t workspace_bytes_for(int n_jobs, long long P, bool deep, int err_stride, b
 }
 
 int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long long NP, bool deep, int err_stride,
-               ws_view& w, bool sort = false) {
+               ws_view& w, bool sort = true) {
     size_t need = workspace_bytes_for(n_jobs, P, deep, err_stride, sort);
     RX_REQUIRE(d_workspace && bytes >= need, RX_ERR_WORKSPACE, "render workspace too small: need %zu bytes, have %zu",
                need, bytes);
@@ -1678,10 +1723,9 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
     tm.s = s;
-    pl.sort_rays = !(params->flags & RX_FLAG_KEEP_QUEUE_ORDER);
     int spp = params->samples_per_pass;
     for (int done = 0; done < n_samples; done += spp) {
-        st = enqueue(w, pl, d_counters, tm, s);
+        st = enqueue(w, pl, d_counters, tm, s, true);
         if (st) return st;
         if (profile) profile->n_passes += 1;
         if (tm.on) {
@@ -1769,7 +1813,7 @@ extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx
             hipGraph_t g;
             hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
             int st = RX_OK;
-            if (e == hipSuccess) st = enqueue(b->w, b->plan, d_counters, off, cap);
+            if (e == hipSuccess) st = enqueue(b->w, b->plan, d_counters, off, cap, false);
             hipError_t e2 = hipStreamEndCapture(cap, &g);
             (void)hipStreamDestroy(cap);
             if (st) return st;
@@ -1790,7 +1834,7 @@ extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx
     tm.on = prof;
     tm.s = s;
     for (int k = 0; k < passes; ++k) {
-        int st = enqueue(b->w, b->plan, d_counters, tm, s);
+        int st = enqueue(b->w, b->plan, d_counters, tm, s, true);
         if (st) return st;
         if (profile) profile->n_passes += 1;
         if (tm.on) {

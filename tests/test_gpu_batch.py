@@ -53,6 +53,34 @@ def test_batch_equals_alone_and_graph_equals_direct():
     b.close()
 
 
+def test_reordered_queues_in_batches_change_nothing():
+    """Batches trace primaries pixel-major and sort their secondary queues
+    too (the whole queue capacity under a HIP graph, idle slots last; the
+    read-back length otherwise): every image equals that of a batch that
+    keeps each queue in path order."""
+    import torch
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd.batch import Batch
+
+    seeds = [4020, 4021, 4022]
+    imgs = {}
+    for name, graph, flags in (("graph", True, 0), ("direct", False, 0),
+                               ("kept", False, nat.RX_FLAG_KEEP_QUEUE_ORDER)):
+        # 3 jobs x 384^2 x 4 samples per pass: above the size whose
+        # queue lengths are read back
+        jobs = _jobs(seeds, res_override=384, spp_pass=4)
+        b = Batch(jobs, use_graph=graph, flags=flags)
+        b.run()
+        torch.cuda.synchronize()
+        imgs[name] = [_color(j) for j in jobs]
+        b.close()
+    for g, d, k in zip(imgs["graph"], imgs["direct"], imgs["kept"]):
+        print(f"mean {k.mean():.4f}: graph==kept {np.array_equal(g, k)} direct==kept {np.array_equal(d, k)}",
+              flush=True)
+        assert np.array_equal(g, k) and np.array_equal(d, k)
+
+
 def test_uneven_jobs_and_oracle_parity():
     """Jobs of different resolutions and sample counts in one batch; each
     matches the oracle's render of that scene."""

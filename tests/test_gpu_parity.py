@@ -344,8 +344,9 @@ def test_reordered_secondary_queues_change_nothing():
     traversal counter equal those of the queue traced in shade's order."""
     from roxy_dcc_amd import _native as nat
 
-    sorted_img, sorted_cnt, sorted_prof = _render_mesh(0)
-    plain_img, plain_cnt, plain_prof = _render_mesh(nat.RX_FLAG_KEEP_QUEUE_ORDER)
+    # 512 x 512 x 4 = 2^20 path slots: large enough to sort
+    sorted_img, sorted_cnt, sorted_prof = _render_mesh(0, 512, 512, 4)
+    plain_img, plain_cnt, plain_prof = _render_mesh(nat.RX_FLAG_KEEP_QUEUE_ORDER, 512, 512, 4)
     print(f"reorder launches {sorted_prof.n_reorder} ({sorted_prof.ms_reorder:.2f} ms); extend "
           f"{sorted_prof.ms_extend:.2f} vs {plain_prof.ms_extend:.2f} ms", flush=True)
     assert sorted_prof.n_reorder >= 3 and plain_prof.n_reorder == 0
