@@ -85,13 +85,15 @@ struct ws_view {
     int2* spill;
     // reordered copy of a ray queue for k_extend (single-job passes): the
     // sort keys and slot indices (double buffers), rocPRIM's scratch;
-    // hit_slot (set per launch) maps a position of the copy back to the
-    // queue slot its hit belongs to (null: the queue itself is traced)
+    // hit_slot (set per launch) maps a position of the traced order back
+    // to the queue slot its hit belongs to, ray_slot (if set) to the slot
+    // its ray is read from (null: position i is slot i)
     gv4f* q_sorted;
     uint32_t *sk_in, *sk_out, *si_in, *si_out;
     void* sort_tmp;
     size_t sort_bytes;
     const uint32_t* hit_slot;
+    const uint32_t* ray_slot;
 };
 
 // Radix-sort scratch for reordering n ray slots (rocPRIM's own query).
@@ -126,6 +128,7 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
                     &w.e_ay, &w.e_az, &w.e_tx, &w.e_ty, &w.e_tz, &w.e_scale};
     for (float** f : pe) *f = ar.take<float>(P);
     w.hit_slot = nullptr;
+    w.ray_slot = nullptr;
     w.q_sorted = nullptr;
     w.sk_in = w.sk_out = w.si_in = w.si_out = nullptr;
     w.sort_tmp = nullptr;
@@ -629,7 +632,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
         {
             if (idle && mine < n) {
                 i = mine;
-                p = get_ray(w, qi, i, ro, rd);
+                p = get_ray(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd);
                 const int jx = job_of_path(w, p);
                 sc = &w.jobs[jx].scene;
                 rays++;
@@ -1412,9 +1415,10 @@ constexpr int PRIM_INLINE_MAX = 256;
 // nearby origins next to each other: the job (most significant, as many
 // bits as the batch needs), the direction quantised to 8 levels per axis
 // (9 bits), and a Morton code of the origin in the job's scene box with
-// the bits left (7 per axis for a single job).  k_extend traces the sorted
-// copy and writes each hit back to its ray's own queue slot, so shading,
-// path state and every result are those of the unsorted queue.
+// the bits left (7 per axis for a single job).  k_extend walks the rays in
+// sorted order, reading each through the sorted slot list, and writes each
+// hit back to its ray's own queue slot, so shading, path state and every
+// result are those of the unsorted queue.
 __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     v &= 0x7fu;
     v = (v | (v << 8)) & 0x0000f00fu;
@@ -1460,17 +1464,6 @@ __global__ void k_ray_keys(ws_view w, int b, long long n, int ob, uint32_t* __re
         }
         key[i] = k;
         idx[i] = (uint32_t)i;
-    }
-}
-
-__global__ void k_ray_gather(ws_view w, int b, long long n, const uint32_t* __restrict__ idx,
-                             gv4f* __restrict__ out) {
-    const long long m = n < (long long)w.counts[b] ? n : (long long)w.counts[b];
-    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < m; i += (long long)gridDim.x * BS) {
-        const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)idx[i];
-        RX_GLOBAL gv4f* o = (RX_GLOBAL gv4f*)out + 2 * i;
-        o[0] = q[0];
-        o[1] = q[1];
     }
 }
 
@@ -1532,8 +1525,10 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
                 RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)n, 0u, 30u, s));
                 rs = tm.end();
                 if (rs) return rs;
-                RX_STAGE(tm, 6, (k_ray_gather<<<gk, BS, 0, s>>>(w, qi, n, slots.current(), w.q_sorted)));
-                we.q_ray[qi] = w.q_sorted;
+                // k_extend reads each ray through the permutation: cheaper
+                // than gathering a sorted copy first (2.6 against 3.7 ms of
+                // reordering per 64-spp mesh render, same traversal time)
+                we.ray_slot = slots.current();
                 we.hit_slot = slots.current();
             }
         }
