@@ -41,6 +41,15 @@ constexpr int MAX_JOBS = 4096;
 // blockIdx % NPART, and are folded into their destination once per pass.
 constexpr int NPART = 64;
 
+// The same holds for handing out queue slots: a persistent kernel's waves
+// take rays from FETCH_SEGS counters, one per contiguous segment of the
+// queue, instead of one, so refills do not queue behind one address.
+#ifndef RX_FETCH_SEGS
+#define RX_FETCH_SEGS 32
+#endif
+constexpr int FETCH_SEGS = RX_FETCH_SEGS;
+constexpr int FETCH_STRIDE = 32;  // ints: 128 bytes between counters
+
 // Adaptive-stop error sums: |change of the running mean| per sample index,
 // summed over pixels and channels in fixed point (2^22 units per colour
 // unit, each term capped at 2^14) so the sum, and the stop decision, do not
@@ -70,6 +79,9 @@ struct ws_view {
     float *e_px, *e_py, *e_pz, *e_nx, *e_ny, *e_nz, *e_ax, *e_ay, *e_az, *e_tx, *e_ty, *e_tz, *e_scale;
     // queue lengths: [0], [1] ray queues, [2] next-event queue
     int32_t* counts;
+    // hand-out counters of the persistent kernels: FETCH_SEGS per kernel
+    // (closest-hit, then shadow), FETCH_STRIDE ints apart
+    int32_t* fetch_seg;
     // per-job queue offsets of the primary rays (n_jobs + 1)
     int32_t* q_off;
     // partial sums of the traversal counters and of each job's rays, one
@@ -110,6 +122,7 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
+    w.fetch_seg = ar.take<int32_t>(2 * FETCH_SEGS * FETCH_STRIDE);
     w.q_off = ar.take<int32_t>(n_jobs + 1);
     w.ctr_part = ar.take<unsigned long long>(NPART * 16);
     w.ray_part = ar.take<unsigned long long>((size_t)NPART * n_jobs);
@@ -276,14 +289,25 @@ __device__ __forceinline__ int wave_cap(int n, int block_size) {
 }
 
 // Hands queue slots to idle lanes: the wave takes up to (cap - busy lanes)
-// consecutive slots from `fetch` with one atomic, in lane order.  Every
-// wave of the launch fetches from the same counter, so a full-width wave
-// waits until REFILL_MIN lanes are free (or all are) before it fetches:
-// a few idle lanes cost less than an atomic per finished ray.  Returns
-// this lane's slot (>= n when it gets none) and sets `exhausted` once the
-// queue has been handed out.
-constexpr int REFILL_MIN = 8;
-__device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool idle, bool& exhausted) {
+// consecutive slots with one atomic, in lane order.  The queue [0, n) is
+// cut into FETCH_SEGS equal segments with a counter each; a wave starts at
+// its home segment and moves to the next one when that runs dry, so
+// every slot is handed out once and the waves' atomics spread over
+// FETCH_SEGS addresses.  A full-width wave still waits until REFILL_MIN
+// lanes are free (or all are) before it fetches.  Returns this lane's
+// slot (>= n when it gets none) and sets `exhausted` once every segment
+// has run dry.  seg / tried: the wave's current segment and how many it
+// has found empty (wave-uniform).
+#ifndef RX_REFILL_MIN
+#define RX_REFILL_MIN 40
+#endif
+constexpr int REFILL_MIN = RX_REFILL_MIN;
+__device__ __forceinline__ int home_segment(int block_size) {
+    return (int)(((long long)blockIdx.x * (block_size / 64) + (threadIdx.x >> 6)) % FETCH_SEGS);
+}
+
+__device__ __forceinline__ int wave_fetch(int32_t* ctr, int n, int cap, bool idle, bool& exhausted, int& seg,
+                                          int& tried) {
     unsigned long long m = __ballot(idle);
     int lane = threadIdx.x & 63;
     int n_idle = __popcll(m);
@@ -291,13 +315,28 @@ __device__ __forceinline__ int wave_fetch(int32_t* fetch, int n, int cap, bool i
     if (want > n_idle) want = n_idle;
     if (want <= 0) return n;
     if (cap >= 32 && want < REFILL_MIN && n_idle < 64) return n;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(fetch, want);
-    base = __shfl(base, 0, 64);
-    if (base + want >= n) exhausted = true;
+    if (n <= 0) tried = FETCH_SEGS;
     unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
     int rank = __popcll(below);
-    return (idle && rank < want) ? base + rank : n;
+    while (tried < FETCH_SEGS) {
+        const long long s0 = (long long)n * seg / FETCH_SEGS, s1 = (long long)n * (seg + 1) / FETCH_SEGS;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&ctr[seg * FETCH_STRIDE], want);
+        base = __shfl(base, 0, 64);
+        const long long avail = s1 - s0 - base;
+        if (avail >= want) {
+            if (avail == want) {
+                seg = seg + 1 == FETCH_SEGS ? 0 : seg + 1;
+                ++tried;
+            }
+            return (idle && rank < want) ? (int)(s0 + base + rank) : n;
+        }
+        seg = seg + 1 == FETCH_SEGS ? 0 : seg + 1;
+        ++tried;
+        if (avail > 0) return (idle && rank < (int)avail) ? (int)(s0 + base + rank) : n;
+    }
+    exhausted = true;
+    return n;
 }
 
 // Appends to a queue with one atomic per block: every thread of the block
@@ -343,6 +382,7 @@ __global__ __launch_bounds__(PB) void k_pass_begin(ws_view w) {
     __shared__ int32_t part[PB];
     const int t = threadIdx.x;
     if (t < 8) w.counts[t] = 0;
+    for (int k = t; k < 2 * FETCH_SEGS; k += PB) w.fetch_seg[k * FETCH_STRIDE] = 0;
     for (int k = t; k < NPART * 16; k += PB) w.ctr_part[k] = 0ull;
     const int per = (w.n_jobs + PB - 1) / PB;
     const int j0 = t * per, j1 = min(w.n_jobs, j0 + per);
@@ -593,7 +633,8 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
 // Persistent closest-hit kernel with per-lane ray refill.  A fixed grid of
 // waves runs until the queue is drained; whenever lanes of a wave have
 // finished their rays, the wave takes as many new queue slots as it has
-// idle lanes with one atomic (counts[3], zeroed before each extend launch)
+// idle lanes with one atomic (wave_fetch over the first FETCH_SEGS
+// counters, zeroed before each extend launch)
 // and hands them out in lane order.  Between refills each busy lane runs
 // its traversal to the next leaf and intersects that leaf, so lanes are
 // never held idle for a whole ray by the slowest ray of the wave.  Each
@@ -606,7 +647,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
     const int n = w.counts[qi];
-    int32_t* fetch = &w.counts[3];
+    int32_t* fetch = w.fetch_seg;
+    int seg = home_segment(SBS), tried = 0;
     const int cap = wave_cap(n, SBS);
     // the queue this bounce's shade fills and the next-event queue start empty
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -628,7 +670,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     bool have = false;
     while (true) {
         bool idle = i < 0;
-        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted);
+        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted, seg, tried);
         {
             if (idle && mine < n) {
                 i = mine;
@@ -781,12 +823,10 @@ __device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
 __global__ __launch_bounds__(BS) void k_shade(ws_view w, int qi, int depth) {
     __shared__ int32_t app[BS / 64 + 1];
     const int n = w.counts[qi];
-    // the next bounce's extend hands out its queue from counts[3], this
-    // bounce's connect from counts[4]
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        w.counts[3] = 0;
-        w.counts[4] = 0;
-    }
+    // the next bounce's extend hands out its queue from the first
+    // FETCH_SEGS counters, this bounce's connect from the others
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < 2 * FETCH_SEGS; k += BS) w.fetch_seg[k * FETCH_STRIDE] = 0;
     const int qo = 1 - qi;
     const int stride = gridDim.x * BS;
     for (int base = xcd_block() * BS; base < n; base += stride) {
@@ -975,7 +1015,8 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // (scale = 1 for diffuse, roughness for glossy, trace.py:199-212).
 //
 // Persistent like k_extend: lanes that finish a request take new ones
-// (counts[4], zeroed by shade).  A lane's request walks its lights in
+// (the second set of FETCH_SEGS counters, zeroed by shade).  A lane's
+// request walks its lights in
 // order (one light in mode "one", all of them in mode "all"); for each it
 // samples the cone, casts the shadow ray if n.l > 0 and max_t > 0.001, and
 // runs the any-hit traversal a leaf at a time between refills.  The
@@ -987,7 +1028,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
                                    (int)gridDim.x * SBS};
     const int n = w.counts[2];
-    int32_t* fetch = &w.counts[4];
+    int32_t* fetch = w.fetch_seg + FETCH_SEGS * FETCH_STRIDE;
+    int seg = home_segment(SBS), tried = 0;
     const int cap = wave_cap(n, SBS);
     const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
@@ -1007,7 +1049,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     float max_t = 0.0f, ndotl = 0.0f, solid = 0.0f;
     while (true) {
         bool idle = i < 0;
-        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted);
+        int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted, seg, tried);
         {
             if (idle && mine < n) {
                 i = mine;
