@@ -12,8 +12,9 @@ per job list, whatever runs fastest on the device at hand:
     resident blocks per CU of the persistent traversal kernels (0 = the
     register budget's maximum).
 ``small_blocks``
-    64-thread traversal blocks with the deep stack instead of 256-thread
-    blocks with the shallow one (block shape and stack depth).
+    64-thread traversal blocks with the deep stack and no LDS treelet
+    instead of the large blocks (1024 / 512 threads) that share one
+    (block shape and stack depth).
 ``prim_prepass``
     closest-primitive scan as its own pass over each ray queue.
 
@@ -143,8 +144,9 @@ class Autotuner:
         for k in pass_candidates(jobs):
             consider(Tuning(passes=k))
         base = best
-        for bpc in (6, 4):
-            consider(dataclasses.replace(base, blocks_per_cu=bpc))
+        # one block per CU instead of the default two closest-hit / three
+        # shadow blocks: half the resident waves, twice the LDS per wave
+        consider(dataclasses.replace(base, blocks_per_cu=1))
         if max(j.scene.builder.depth for j in jobs) + 1 <= 24:
             # 64-thread blocks: occupancy back to their own default
             consider(dataclasses.replace(best, small_blocks=True, blocks_per_cu=0))

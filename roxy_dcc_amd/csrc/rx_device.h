@@ -418,12 +418,8 @@ struct lane_stack {
 // stack round trip fewer per node.  Returns false when neither child box
 // is hit (the caller pops).
 template <class Stack>
-__device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
-                                               float limit, Stack& stk, int& sp, int& next) {
-    // the two child boxes and the two links: 56 of the record's 64 bytes
-    const RX_GLOBAL gv4f* p = (const RX_GLOBAL gv4f*)(nodes + link);
-    gv4f a = p[0], b = p[1], c = p[2];
-    gv2i d = ((const RX_GLOBAL gv2i*)p)[6];
+__device__ __forceinline__ bool visit_boxes(gv4f a, gv4f b, gv4f c, gv2i d, f3 ro, f3 inv, float limit, Stack& stk,
+                                            int& sp, int& next) {
     float lt, rt;
     bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
     bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
@@ -446,6 +442,29 @@ __device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes
         return true;
     }
     return false;
+}
+
+template <class Stack>
+__device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
+                                               float limit, Stack& stk, int& sp, int& next) {
+    // the two child boxes and the two links: 56 of the record's 64 bytes
+    const RX_GLOBAL gv4f* p = (const RX_GLOBAL gv4f*)(nodes + link);
+    return visit_boxes(p[0], p[1], p[2], ((const RX_GLOBAL gv2i*)p)[6], ro, inv, limit, stk, sp, next);
+}
+
+// Treelet links: node indices stay below TL_REF; TL_REF + k names slot k of
+// a block's LDS copy of the top of the tree (at most TL_NODES slots): the
+// node's two child boxes as three 16-byte words at box[3 k ..] and its
+// links, already rewritten to slots where the child has one, at link[k].
+// The same boxes, links and tests as the node record, so the visit
+// sequence is visit_interior's.
+constexpr int TL_NODES = 511;
+constexpr int TL_REF = 1 << 30;
+
+template <class Stack>
+__device__ __forceinline__ bool visit_treelet(const RX_LDS gv4f* box, const RX_LDS gv2i* link, int k, f3 ro, f3 inv,
+                                              float limit, Stack& stk, int& sp, int& next) {
+    return visit_boxes(box[3 * k], box[3 * k + 1], box[3 * k + 2], link[k], ro, inv, limit, stk, sp, next);
 }
 
 // Pops the next entry whose t_enter does not exceed `limit` (entries

@@ -61,6 +61,42 @@ inline int trav_grid(long long n, bool deep, int per_cu = 0) {
     int pc = per_cu <= 0 || per_cu > lim ? lim : per_cu;
     return grid_for(n, deep ? BS_DEEP : BS_SHALLOW, pc);
 }
+// The render's traversal kernels on shallow trees (rx_render.hip) keep
+// the top of the pass's BVH in LDS: the first TL_NODES interior nodes in
+// breadth-first order (a treelet, 56 B each), which take about half of a
+// ray's interior-node fetches on a 70k-triangle mesh and a third on a
+// 2M-triangle scene, off the vector-memory path.  Links into the treelet
+// are TL_REF + slot (node indices stay below TL_REF).  One copy serves a
+// whole block, so the blocks are large: 1024 threads for the closest-hit
+// kernel (2 per CU at 8 waves per SIMD), 512 for the shadow kernel (3 per
+// CU at 6), with STACK_LDS_TL stack entries per lane in LDS:
+// 1024 x 6 x 8 B + 511 x 56 B = 76.6 KB and 512 x 6 x 8 B + 28.6 KB =
+// 52.6 KB, within a CU's 160 KB at those block counts (TL_NODES and
+// TL_REF are in rx_device.h).
+constexpr int STACK_LDS_TL = 6;
+constexpr int BS_EXTEND = 1024;
+constexpr int EXTEND_BLOCKS_PER_CU = 2;
+constexpr int BS_CONNECT = 512;
+constexpr int CONNECT_BLOCKS_PER_CU = 3;
+
+// Grid of a render traversal kernel with `bs`-thread blocks, `lim` of
+// which fit a CU (per_cu clamped to 1 .. lim as in trav_grid).
+inline int render_trav_grid(long long n, int bs, int lim, int per_cu) {
+    int pc = per_cu <= 0 || per_cu > lim ? lim : per_cu;
+    return grid_for(n, bs, pc);
+}
+
+// Spill area of the render kernels: the largest resident grid of either
+// stack class times the entries a lane may spill.
+inline size_t render_spill_entries(bool deep) {
+    long long cu = device_cu_count();
+    if (deep)
+        return (size_t)(STACK_DEEP - STACK_LDS) * (size_t)(cu * TRAV_BLOCKS_PER_CU_DEEP * BS_DEEP + 8LL * BS_DEEP);
+    long long ext = cu * EXTEND_BLOCKS_PER_CU * BS_EXTEND + 8LL * BS_EXTEND;
+    long long con = cu * CONNECT_BLOCKS_PER_CU * BS_CONNECT + 8LL * BS_CONNECT;
+    return (size_t)(STACK_SHALLOW - STACK_LDS_TL) * (size_t)(ext > con ? ext : con);
+}
+
 inline size_t trav_spill_entries(bool deep) {
     long long threads = deep ? (long long)device_cu_count() * TRAV_BLOCKS_PER_CU_DEEP * BS_DEEP
                              : (long long)device_cu_count() * TRAV_BLOCKS_PER_CU_SHALLOW * BS_SHALLOW;

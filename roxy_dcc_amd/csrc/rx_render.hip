@@ -106,6 +106,14 @@ struct ws_view {
     size_t sort_bytes;
     const uint32_t* hit_slot;
     const uint32_t* ray_slot;
+    // top of the tree for the traversal kernels' LDS (k_treelet): slot k's
+    // child boxes at tl_box[3 k ..], its links at tl_link[k], the slot
+    // count at tl_count[0]; built for the BVH whose nodes are tl_nodes
+    // (set per pass by the host; null: no treelet)
+    gv4f* tl_box;
+    gv2i* tl_link;
+    int32_t* tl_count;
+    const rx_node* tl_nodes;
 };
 
 // Radix-sort scratch for reordering n ray slots (rocPRIM's own query).
@@ -118,7 +126,11 @@ size_t ray_sort_bytes(long long n) {
 
 void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
     w.jobs = ar.take<rx_job>(n_jobs);
-    w.spill = ar.take<int2>(trav_spill_entries(deep));
+    w.spill = ar.take<int2>(render_spill_entries(deep));
+    w.tl_box = ar.take<gv4f>(3 * TL_NODES);
+    w.tl_link = ar.take<gv2i>(TL_NODES);
+    w.tl_count = ar.take<int32_t>(4);
+    w.tl_nodes = nullptr;
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
@@ -630,6 +642,88 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
     if (COUNT) wave_add(&ctr_slot(w)->prim_tests, cnt.prims);
 }
 
+// ---------------------------------------------------------------- treelet
+// The top of the pass's BVH for the traversal kernels' LDS: the first
+// TL_NODES interior nodes in breadth-first order, root in slot 0, each as
+// its two child boxes and two links, where the link of an interior child
+// that also has a slot becomes TL_REF + slot.  Level by level: one thread
+// per node of the level; the level's interior children take the next free
+// slots in level order (exclusive scan over the level's child counts).
+// Links to leaves and to interior nodes past the last slot stay as they
+// are.  One block; a level never holds more than TL_NODES < TLB nodes.
+constexpr int TLB = 512;
+static_assert(TL_NODES < TLB, "a treelet level must fit one block");
+__global__ __launch_bounds__(TLB) void k_treelet(const rx_node* __restrict__ nodes,
+                                                 const rx_bvh_header* __restrict__ header, ws_view w) {
+    __shared__ int src[TL_NODES];
+    __shared__ int scan[TLB];
+    const int tid = threadIdx.x;
+    const int root = gload(header).root_link;
+    if (tid == 0) src[0] = root;
+    __syncthreads();
+    int lo = 0, hi = root >= 0 ? 1 : 0;
+    while (lo < hi) {
+        const int s = lo + tid;
+        const bool mine = s < hi;
+        gv4f a{0.0f, 0.0f, 0.0f, 0.0f}, b = a, c = a;
+        gv2i d{-1, -1};
+        if (mine) {
+            const RX_GLOBAL gv4f* p = (const RX_GLOBAL gv4f*)(nodes + src[s]);
+            a = p[0];
+            b = p[1];
+            c = p[2];
+            d = ((const RX_GLOBAL gv2i*)p)[6];
+        }
+        const int kids = mine ? (d.x >= 0 ? 1 : 0) + (d.y >= 0 ? 1 : 0) : 0;
+        scan[tid] = kids;
+        __syncthreads();
+        for (int off = 1; off < TLB; off <<= 1) {
+            int v = tid >= off ? scan[tid - off] : 0;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        const int total = scan[TLB - 1];
+        if (mine) {
+            int k = hi + scan[tid] - kids;
+            gv2i e = d;
+            if (d.x >= 0) {
+                if (k < TL_NODES) {
+                    src[k] = d.x;
+                    e.x = TL_REF + k;
+                }
+                ++k;
+            }
+            if (d.y >= 0 && k < TL_NODES) {
+                src[k] = d.y;
+                e.y = TL_REF + k;
+            }
+            RX_GLOBAL gv4f* o = (RX_GLOBAL gv4f*)w.tl_box + 3 * s;
+            o[0] = a;
+            o[1] = b;
+            o[2] = c;
+            ((RX_GLOBAL gv2i*)w.tl_link)[s] = e;
+        }
+        __syncthreads();
+        lo = hi;
+        hi = hi + total < TL_NODES ? hi + total : TL_NODES;
+    }
+    if (tid == 0) w.tl_count[0] = hi;
+}
+
+// A block's LDS copy of the treelet (TLN = 0: none).  Returns the slot
+// count, 0 when the pass has no treelet.
+template <int SBS, int TLN>
+__device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* link) {
+    static_assert(TLN == 0 || TLN == TL_NODES, "the treelet's links address TL_NODES slots");
+    if (TLN == 0 || w.tl_nodes == nullptr) return 0;
+    const int n = w.tl_count[0];
+    for (int k = threadIdx.x; k < 3 * n; k += SBS) box[k] = ((const RX_GLOBAL gv4f*)w.tl_box)[k];
+    for (int k = threadIdx.x; k < n; k += SBS) link[k] = ((const RX_GLOBAL gv2i*)w.tl_link)[k];
+    __syncthreads();
+    return n;
+}
+
 // ---------------------------------------------------------------- extend
 // Persistent closest-hit kernel with per-lane ray refill.  A fixed grid of
 // waves runs until the queue is drained; whenever lanes of a wave have
@@ -642,11 +736,16 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
 // ray's own sequence of pops, tests and pushes is the one of
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
-template <int SBS, bool COUNT, bool PRE>
+template <int SBS, int TLN, bool COUNT, bool PRE>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi) {
-    __shared__ int2 stack_mem[STACK_LDS * SBS];
-    lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
-                                   (int)gridDim.x * SBS};
+    constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
+    __shared__ int2 stack_mem[SL * SBS];
+    __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
+    __shared__ gv2i tl_link_mem[TLN > 0 ? TLN : 1];
+    lane_stack<SBS, SL> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x, (int)gridDim.x * SBS};
+    const int tl_n = load_treelet<SBS, TLN>(w, tl_box_mem, tl_link_mem);
+    const RX_LDS gv4f* tl_box = (const RX_LDS gv4f*)tl_box_mem;
+    const RX_LDS gv2i* tl_link = (const RX_LDS gv2i*)tl_link_mem;
     const int n = w.counts[qi];
     int32_t* fetch = w.fetch_seg;
     int seg = home_segment(SBS), tried = 0;
@@ -702,7 +801,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     if (slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2],
                              best_t, &rt)) {
                         have = true;
-                        cur = hdr.root_link;
+                        cur = TLN > 0 && tl_n > 0 && nodes == w.tl_nodes ? TL_REF : hdr.root_link;
                         sp = 0;
                     }
                 }
@@ -718,8 +817,22 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     if (inner_done(n_in, entered, it)) break;
                     if (in) {
                         if (COUNT) cnt.pops++;
-                        if (!visit_interior(nodes, cur, ro, inv, best_t, stk, sp, cur))
-                            have = pop_next(stk, sp, best_t, cur);
+                        gv4f a, b, c;
+                        gv2i dl;
+                        if (TLN > 0 && cur >= TL_REF) {
+                            const int k = cur - TL_REF;
+                            a = tl_box[3 * k];
+                            b = tl_box[3 * k + 1];
+                            c = tl_box[3 * k + 2];
+                            dl = tl_link[k];
+                        } else {
+                            const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)(nodes + cur);
+                            a = q[0];
+                            b = q[1];
+                            c = q[2];
+                            dl = ((const RX_GLOBAL gv2i*)q)[6];
+                        }
+                        if (!visit_boxes(a, b, c, dl, ro, inv, best_t, stk, sp, cur)) have = pop_next(stk, sp, best_t, cur);
                     }
                 }
                 if (have && cur < 0) {
@@ -1029,11 +1142,16 @@ __device__ __forceinline__ void sphere_light_sample(f3 hp, f3 c, float radius, f
 // runs the any-hit traversal a leaf at a time between refills.  The
 // per-request order of lights and the sum order of `direct` are the
 // reference's.
-template <int SBS, bool COUNT>
+template <int SBS, int TLN, bool COUNT>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAVES_PER_EU))) void k_connect(ws_view w, int depth) {
-    __shared__ int2 stack_mem[STACK_LDS * SBS];
-    lane_stack<SBS, STACK_LDS> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x,
-                                   (int)gridDim.x * SBS};
+    constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
+    __shared__ int2 stack_mem[SL * SBS];
+    __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
+    __shared__ gv2i tl_link_mem[TLN > 0 ? TLN : 1];
+    lane_stack<SBS, SL> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x, (int)gridDim.x * SBS};
+    const int tl_n = load_treelet<SBS, TLN>(w, tl_box_mem, tl_link_mem);
+    const RX_LDS gv4f* tl_box = (const RX_LDS gv4f*)tl_box_mem;
+    const RX_LDS gv2i* tl_link = (const RX_LDS gv2i*)tl_link_mem;
     const int n = w.counts[2];
     int32_t* fetch = w.fetch_seg + FETCH_SEGS * FETCH_STRIDE;
     int seg = home_segment(SBS), tried = 0;
@@ -1115,7 +1233,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                         if (slab(so, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1],
                                  hdr.bmax[2], max_t, &rt)) {
                             have = true;
-                            cur = hdr.root_link;
+                            cur = TLN > 0 && tl_n > 0 && J->scene.nodes == w.tl_nodes ? TL_REF : hdr.root_link;
                             sp = 0;
                         }
                     }
@@ -1133,8 +1251,22 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                     if (inner_done(n_in, entered, it)) break;
                     if (in) {
                         if (COUNT) cnt.pops++;
-                        if (!visit_interior(nodes, cur, so, inv, max_t, stk, sp, cur))
-                            have = pop_next(stk, sp, max_t, cur);
+                        gv4f a, b, c;
+                        gv2i dl;
+                        if (TLN > 0 && cur >= TL_REF) {
+                            const int k = cur - TL_REF;
+                            a = tl_box[3 * k];
+                            b = tl_box[3 * k + 1];
+                            c = tl_box[3 * k + 2];
+                            dl = tl_link[k];
+                        } else {
+                            const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)(nodes + cur);
+                            a = q[0];
+                            b = q[1];
+                            c = q[2];
+                            dl = ((const RX_GLOBAL gv2i*)q)[6];
+                        }
+                        if (!visit_boxes(a, b, c, dl, so, inv, max_t, stk, sp, cur)) have = pop_next(stk, sp, max_t, cur);
                     }
                 }
                 if (have && cur < 0) {
@@ -1446,6 +1578,10 @@ struct pass_plan {
     int max_spp;             // most samples per pass of any job
     bool adaptive;           // some job has RX_FLAG_ADAPTIVE
     bool sort_rays;          // reorder ray queues for k_extend (no job asks to keep the order)
+    // BVH whose top the traversal kernels keep in LDS: the one of the job
+    // with the most paths per pass (null: no triangles)
+    const rx_node* tl_nodes;
+    const rx_bvh_header* tl_header;
 };
 
 // Scenes with more analytic primitives than this get their closest-
@@ -1456,6 +1592,12 @@ struct pass_plan {
 // kernel waits on node fetches, not on arithmetic), hence the high bar;
 // RX_FLAG_PRIM_PREPASS asks for the pre-pass at any count.
 constexpr int PRIM_INLINE_MAX = 256;
+
+// Passes of fewer path slots than this run the 64-thread traversal blocks
+// without a treelet: building and loading the treelet, and filling a CU
+// with two 1024-thread blocks, do not pay off on a few hundred thousand
+// rays (a loop of one-sample passes over small frames).
+constexpr long long SMALL_PASS_PATHS = 1 << 19;
 
 // ---------------------------------------------------------------- reorder
 // Secondary rays leave in scattered directions, so lanes of one wave walk
@@ -1535,13 +1677,24 @@ inline int key_origin_bits(int n_jobs) {
 // host_counts: the caller may synchronise (not under graph capture), so
 // each queue's length is read back and only that many rays are sorted;
 // otherwise the whole capacity is sorted, idle slots last.
-template <int SBS>
+// Deep trees: 64-thread blocks, the stack mostly in LDS, no treelet.
+// Shallow trees: the large blocks that share one LDS treelet (measured on
+// the mesh: 2074 Mrays/s without it, 2168 with it; the 2M-triangle scene
+// is unchanged).
+template <bool DEEP>
 int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
                  bool host_counts) {
+    constexpr int EBS = DEEP ? BS_DEEP : BS_EXTEND;
+    constexpr int CBS = DEEP ? BS_DEEP : BS_CONNECT;
+    constexpr int TLN = DEEP ? 0 : TL_NODES;
+    int ge = DEEP ? trav_grid(w0.P, true, pl.trav_per_cu)
+                  : render_trav_grid(w0.P, BS_EXTEND, EXTEND_BLOCKS_PER_CU, pl.trav_per_cu);
+    int gc = DEEP ? ge : render_trav_grid(w0.P, BS_CONNECT, CONNECT_BLOCKS_PER_CU, pl.trav_per_cu);
     ws_view w = w0;
+    w.tl_nodes = TLN > 0 ? pl.tl_nodes : nullptr;
     int gp = grid_for(w.P, BS, 16);
-    int gt = trav_grid(w.P, SBS == BS_DEEP, pl.trav_per_cu);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
+    if (w.tl_nodes) RX_STAGE(tm, 0, (k_treelet<<<1, TLB, 0, s>>>(pl.tl_nodes, pl.tl_header, w)));
     // passes of several samples per pixel trace their primary rays
     // pixel-major (k_generate_pm); later bounces are sorted below
     const bool pm = pl.sort_rays && w.q_sorted && pl.max_spp > 1;
@@ -1584,22 +1737,22 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
         if (pl.prim_prepass) {
             if (pl.count) {
                 RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<SBS, true, true><<<gt, SBS, 0, s>>>(we, qi)));
+                RX_STAGE(tm, 1, (k_extend<EBS, TLN, true, true><<<ge, EBS, 0, s>>>(we, qi)));
             } else {
                 RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<SBS, false, true><<<gt, SBS, 0, s>>>(we, qi)));
+                RX_STAGE(tm, 1, (k_extend<EBS, TLN, false, true><<<ge, EBS, 0, s>>>(we, qi)));
             }
         } else if (pl.count) {
-            RX_STAGE(tm, 1, (k_extend<SBS, true, false><<<gt, SBS, 0, s>>>(we, qi)));
+            RX_STAGE(tm, 1, (k_extend<EBS, TLN, true, false><<<ge, EBS, 0, s>>>(we, qi)));
         } else {
-            RX_STAGE(tm, 1, (k_extend<SBS, false, false><<<gt, SBS, 0, s>>>(we, qi)));
+            RX_STAGE(tm, 1, (k_extend<EBS, TLN, false, false><<<ge, EBS, 0, s>>>(we, qi)));
         }
         RX_STAGE(tm, 2, (k_shade<<<grid_for(w.P, SHADE_BS, 16 * BS / SHADE_BS), SHADE_BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             if (pl.count)
-                RX_STAGE(tm, 3, (k_connect<SBS, true><<<gt, SBS, 0, s>>>(w, d)));
+                RX_STAGE(tm, 3, (k_connect<CBS, TLN, true><<<gc, CBS, 0, s>>>(w, d)));
             else
-                RX_STAGE(tm, 3, (k_connect<SBS, false><<<gt, SBS, 0, s>>>(w, d)));
+                RX_STAGE(tm, 3, (k_connect<CBS, TLN, false><<<gc, CBS, 0, s>>>(w, d)));
         }
     }
     if (pl.adaptive) RX_STAGE(tm, 4, (k_adapt_err<<<grid_for(w.NP, BS, 16), BS, 0, s>>>(w)));
@@ -1611,8 +1764,8 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
 
 int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
             bool host_counts) {
-    if (pl.deep) return enqueue_pass<BS_DEEP>(w, pl, ctr, tm, s, host_counts);
-    return enqueue_pass<BS_SHALLOW>(w, pl, ctr, tm, s, host_counts);
+    if (pl.deep) return enqueue_pass<true>(w, pl, ctr, tm, s, host_counts);
+    return enqueue_pass<false>(w, pl, ctr, tm, s, host_counts);
 }
 
 int check_job(const rx_job& j, int idx) {
@@ -1649,6 +1802,9 @@ int check_job(const rx_job& j, int idx) {
                    "job %d: scene has triangles but no BVH/shading arrays", idx);
         RX_REQUIRE(sc.bvh_depth >= 0 && sc.bvh_depth <= RX_MAX_BVH_DEPTH, RX_ERR_CAPACITY, "BVH depth %d exceeds %d",
                    sc.bvh_depth, RX_MAX_BVH_DEPTH);
+        // node links share their range with the treelet slots from TL_REF up
+        RX_REQUIRE(sc.n_nodes >= 0 && sc.n_nodes < TL_REF, RX_ERR_CAPACITY, "job %d: BVH of %d nodes exceeds 2^30 - 1",
+                   idx, sc.n_nodes);
     }
     return RX_OK;
 }
@@ -1658,7 +1814,8 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
               int32_t* pix_off) {
     RX_REQUIRE(jobs && n >= 1 && n <= MAX_JOBS, RX_ERR_INVALID, "need 1..%d jobs", MAX_JOBS);
     long long p = 0, q = 0;
-    pass_plan plan{0, 0, false, false, false, 0, 0, false, true};
+    pass_plan plan{0, 0, false, false, false, 0, 0, false, true, nullptr, nullptr};
+    long long tl_paths = 0;
     for (int i = 0; i < n; ++i) {
         int st = check_job(jobs[i], i);
         if (st) return st;
@@ -1678,10 +1835,16 @@ int plan_jobs(const rx_job* jobs, int n, long long* P, long long* NP, pass_plan*
         if (j.params.flags & RX_FLAG_ADAPTIVE) plan.adaptive = true;
         if (j.params.flags & RX_FLAG_COUNT_TRAFFIC) plan.count = true;
         if (j.params.flags & RX_FLAG_KEEP_QUEUE_ORDER) plan.sort_rays = false;
+        if (j.scene.n_tris > 0 && npix * j.params.samples_per_pass > tl_paths) {
+            tl_paths = npix * j.params.samples_per_pass;
+            plan.tl_nodes = j.scene.nodes;
+            plan.tl_header = j.scene.header;
+        }
         if (j.scene.n_prims > 0 && (j.scene.n_prims > PRIM_INLINE_MAX || (j.params.flags & RX_FLAG_PRIM_PREPASS)))
             plan.prim_prepass = true;
     }
     RX_REQUIRE(p <= 0x7fffffffLL, RX_ERR_CAPACITY, "batch of %lld paths exceeds 2^31-1", p);
+    if (p < SMALL_PASS_PATHS) plan.deep = true;
     if (path_off) path_off[n] = (int32_t)p;
     if (pix_off) pix_off[n] = (int32_t)q;
     *P = p;
@@ -1730,7 +1893,7 @@ extern "C" int rx_render_workspace_size(int width, int height, int samples_per_p
                "width, height and samples_per_pass must be positive");
     long long P = (long long)width * height * samples_per_pass;
     RX_REQUIRE(P <= 0x7fffffffLL, RX_ERR_CAPACITY, "pass of %lld paths exceeds 2^31-1", P);
-    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth), samples_per_pass, true);
+    *bytes = workspace_bytes_for(1, P, use_deep_stack(bvh_depth) || P < SMALL_PASS_PATHS, samples_per_pass, true);
     return RX_OK;
 }
 

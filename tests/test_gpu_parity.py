@@ -296,7 +296,11 @@ def test_any_hit_matches_oracle():
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("name,size,spp", [("cornell", 48, 8), ("mesh", 40, 4)])
+# the last two are passes of 2^19+ path slots: the 1024-thread traversal
+# blocks with the LDS treelet, pixel-major primaries and sorted secondary
+# queues (smaller passes run the 64-thread shape)
+@pytest.mark.parametrize("name,size,spp", [("cornell", 48, 8), ("mesh", 40, 4), ("mesh", 512, 2),
+                                           ("large", 768, 1)])
 def test_render_matches_oracle(name, size, spp):
     res = helpers.render_pair(name, width=size, height=size, samples=spp)
     g, c = res["gpu_color"], res["cpu_color"]

@@ -103,9 +103,15 @@ def test_render_workspace_size_grows_with_paths():
     # 49 four-byte words per path, plus the fixed traversal spill area
     assert ws2.value - ws1.value >= 3 * 64 * 64 * 49 * 4
     assert ws1.value >= 64 * 64 * 49 * 4
-    wsd = C.c_size_t()
-    assert nat.lib().rx_render_workspace_size(64, 64, 1, 40, C.byref(wsd)) == 0
-    assert wsd.value > ws1.value  # deep trees spill more stack entries
+    # deep trees spill more stack entries (passes of 2^19 path slots or
+    # more; smaller passes always run the deep-stack shape)
+    wss, wsd = C.c_size_t(), C.c_size_t()
+    assert nat.lib().rx_render_workspace_size(1024, 1024, 1, 0, C.byref(wss)) == 0
+    assert nat.lib().rx_render_workspace_size(1024, 1024, 1, 40, C.byref(wsd)) == 0
+    assert wsd.value > wss.value
+    ws_small_deep = C.c_size_t()
+    assert nat.lib().rx_render_workspace_size(64, 64, 1, 40, C.byref(ws_small_deep)) == 0
+    assert ws_small_deep.value == ws1.value
 
 
 def test_struct_layouts_match_header(tmp_path):
