@@ -120,7 +120,8 @@ struct ws_view {
 size_t ray_sort_bytes(long long n) {
     size_t sb = 0;
     rocprim::double_buffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
-    if (rocprim::radix_sort_pairs(nullptr, sb, k, v, (unsigned int)(n > 0 ? n : 1), 0u, 30u) != hipSuccess) return 0;
+    // sized for the widest key (32 bits); sorts of fewer bits need no more
+    if (rocprim::radix_sort_pairs(nullptr, sb, k, v, (unsigned int)(n > 0 ? n : 1), 0u, 32u) != hipSuccess) return 0;
     return sb;
 }
 
@@ -1666,13 +1667,21 @@ __global__ void k_ray_keys(ws_view w, int b, long long n, int ob, uint32_t* __re
 constexpr int SORT_MIN_RAYS = 8192;
 constexpr long long SORT_MIN_PATHS = 1 << 20;
 
-// Origin bits per axis of the sort key for a batch of n_jobs.
-inline int key_origin_bits(int n_jobs) {
+// Origin bits per axis of the sort key for a batch of n_jobs, and the
+// key's width: job bits, 9 direction bits, 3 origin bits per axis.
+#ifndef RX_KEY_ORIGIN_MAX
+#define RX_KEY_ORIGIN_MAX 7
+#endif
+inline int key_job_bits(int n_jobs) {
     int jb = 0;
     while ((1 << jb) < n_jobs) ++jb;
-    int ob = (30 - 9 - jb) / 3;
-    return ob > 7 ? 7 : (ob < 0 ? 0 : ob);
+    return jb;
 }
+inline int key_origin_bits(int n_jobs) {
+    int ob = (30 - 9 - key_job_bits(n_jobs)) / 3;
+    return ob > RX_KEY_ORIGIN_MAX ? RX_KEY_ORIGIN_MAX : (ob < 0 ? 0 : ob);
+}
+inline unsigned key_bits(int n_jobs) { return (unsigned)(key_job_bits(n_jobs) + 9 + 3 * key_origin_bits(n_jobs)); }
 
 // host_counts: the caller may synchronise (not under graph capture), so
 // each queue's length is read back and only that many rays are sorted;
@@ -1724,7 +1733,8 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
                 size_t sb = w.sort_bytes;
                 int rs = tm.begin(6);
                 if (rs) return rs;
-                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)n, 0u, 30u, s));
+                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)n, 0u,
+                                                     key_bits(w.n_jobs), s));
                 rs = tm.end();
                 if (rs) return rs;
                 // k_extend reads each ray through the permutation: cheaper
