@@ -315,8 +315,20 @@ __device__ __forceinline__ int wave_cap(int n, int block_size) {
 #define RX_REFILL_MIN 40
 #endif
 constexpr int REFILL_MIN = RX_REFILL_MIN;
+#ifndef RX_XCD_SEGMENTS
+#define RX_XCD_SEGMENTS 1
+#endif
 __device__ __forceinline__ int home_segment(int block_size) {
-    return (int)(((long long)blockIdx.x * (block_size / 64) + (threadIdx.x >> 6)) % FETCH_SEGS);
+    const int wpb = block_size / 64, wave = threadIdx.x >> 6;
+    if (RX_XCD_SEGMENTS && FETCH_SEGS % 8 == 0) {
+        // blocks go to the 8 XCDs round-robin (block b on XCD b % 8): the
+        // waves of one XCD start in its own eighth of the segments, so each
+        // XCD's L2 holds the nodes of one stretch of the (sorted) queue
+        constexpr int per = FETCH_SEGS / 8;
+        const long long k = (long long)(blockIdx.x >> 3) * wpb + wave;
+        return (blockIdx.x & 7) * per + (int)(k % per);
+    }
+    return (int)(((long long)blockIdx.x * wpb + wave) % FETCH_SEGS);
 }
 
 __device__ __forceinline__ int wave_fetch(int32_t* ctr, int n, int cap, bool idle, bool& exhausted, int& seg,
