@@ -185,6 +185,11 @@ __device__ __forceinline__ int owner(const int32_t* __restrict__ off, int n, lon
     return lo;
 }
 
+// Set bits of the wave mask m in the lanes below this one (v_mbcnt).
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 __device__ __forceinline__ int job_of_path(const ws_view& w, int p) { return w.n_jobs == 1 ? 0 : w.p_job[p]; }
 
 // Ray queue records: two 16-byte words per slot, so a ray is one pair of
@@ -261,8 +266,7 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     int base = 0;
     if (lane == 0 && total > 0) base = atomicAdd(counter, total);
     base = __shfl(base, 0, 64);
-    unsigned long long below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    return base + __popcll(below);
+    return base + lanes_below(m);
 }
 
 // End of the interior phase of a traversal step.  Lanes walk interior
@@ -341,8 +345,7 @@ __device__ __forceinline__ int wave_fetch(int32_t* ctr, int n, int cap, bool idl
     if (want <= 0) return n;
     if (cap >= 32 && want < REFILL_MIN && n_idle < 64) return n;
     if (n <= 0) tried = FETCH_SEGS;
-    unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    int rank = __popcll(below);
+    int rank = lanes_below(m);
     while (tried < FETCH_SEGS) {
         const long long s0 = (long long)n * seg / FETCH_SEGS, s1 = (long long)n * (seg + 1) / FETCH_SEGS;
         int base = 0;
@@ -383,8 +386,7 @@ __device__ __forceinline__ int block_append(int32_t* counter, bool want, int32_t
         lds[SB / 64] = tot > 0 ? atomicAdd(counter, tot) : 0;
     }
     __syncthreads();
-    unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    int r = lds[SB / 64] + lds[wv] + __popcll(below);
+    int r = lds[SB / 64] + lds[wv] + lanes_below(m);
     __syncthreads();
     return r;
 }
