@@ -131,3 +131,74 @@ def test_primitive_wins_a_tie_with_the_bvh():
     g, *_ = _trace(_world([[[-1, -1, 1], [1, -1, 1], [0, 1, 1]]], prims=[(Sphere(1.0), (0.0, 0.0, 2.0))]),
                    _rays([[0, 0, 0]], [[0, 0, 1]]))
     assert g["kind"][0] == 1 and g["t"][0] == 1.0
+
+
+def _tie_grid(n=12, copies=3, seed=0):
+    """An n x n grid of unit quads in the plane z = 1 (two triangles each),
+    every triangle present `copies` times, plus each quad's triangle over the
+    other diagonal, overlapping both: many hits tie exactly in t, so which
+    triangle wins is decided by the visit order alone."""
+    rng = np.random.default_rng(seed)
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = [i, j, 1], [i + 1, j, 1], [i + 1, j + 1, 1], [i, j + 1, 1]
+            for _ in range(copies):
+                tris += [[a, b, c], [a, c, d]]
+            tris += [[a, b, d]]
+    tris = np.asarray(tris, np.float64)
+    return tris[rng.permutation(len(tris))]
+
+
+def test_exact_ties_follow_the_reference_visit_order():
+    """Rays straight down the z axis through quad centres, edges and
+    vertices of the grid, and oblique rays: kind, id, t, u and v all equal
+    the oracle's, whose traversal repeats intersect.py:81-152 step by step
+    (strict 0.001 < t < closest keeps the first triangle found at a tied t)."""
+    n = 12
+    xs = np.concatenate([np.arange(n + 1, dtype=np.float64), np.arange(n) + 0.5, np.arange(n) + 0.25])
+    gx, gy = np.meshgrid(xs, xs)
+    o = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, -2.0)], 1)
+    d = np.tile([0.0, 0.0, 1.0], (len(o), 1))
+    rng = np.random.default_rng(3)
+    o2 = np.stack([rng.uniform(0, n, 4000), rng.uniform(0, n, 4000), np.full(4000, -2.0)], 1)
+    d2 = np.stack([rng.normal(size=4000) * 0.2, rng.normal(size=4000) * 0.2, np.ones(4000)], 1)
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    rays = _rays(np.concatenate([o, o2]), np.concatenate([d, d2]))
+    g, o_ = _trace(_world(_tie_grid(n)), rays)
+    hit = o_["kind"] == 2
+    assert hit.mean() > 0.9
+    np.testing.assert_array_equal(g["u"][hit].view(np.uint32), o_["u"][hit].view(np.uint32))
+    np.testing.assert_array_equal(g["v"][hit].view(np.uint32), o_["v"][hit].view(np.uint32))
+
+
+def test_axis_aligned_rays_take_the_large_inverse():
+    """Direction components of magnitude <= 1e-10 use 1e30 as their
+    inverse (intersect.py:17-25); rays along each axis, both signs, and with
+    tiny components, against a closed box of triangles around them."""
+    box = []
+    for axis in range(3):
+        for side in (-1.0, 1.0):
+            u, v = [k for k in range(3) if k != axis]
+            corners = []
+            for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                p = [0.0, 0.0, 0.0]
+                p[axis], p[u], p[v] = side, a, b
+                corners.append(p)
+            box += [[corners[0], corners[1], corners[2]], [corners[0], corners[2], corners[3]]]
+    dirs = []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            for tiny in (0.0, 1e-11, -1e-11, 1e-9):
+                dv = [tiny, tiny, tiny]
+                dv[axis] = sgn
+                dirs.append(dv)
+    dirs = np.asarray(dirs, np.float64)
+    origins = np.asarray([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1], [0.999, 0.0, 0.0]], np.float64)
+    o = np.repeat(origins, len(dirs), axis=0)
+    d = np.tile(dirs, (len(origins), 1))
+    g, o_ = _trace(_world(box), _rays(o, d))
+    # from x = 0.999 along +x the wall is 0.001 away: not past the 0.001
+    # epsilon, so those four rays miss; every other ray hits a wall
+    near = (o[:, 0] == 0.999) & (d[:, 0] > 0.5)
+    assert (o_["kind"][near] == 0).all() and (o_["kind"][~near] == 2).all()
