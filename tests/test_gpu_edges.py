@@ -137,3 +137,29 @@ def test_negative_ray_count_is_an_error():
     assert st != 0
     with pytest.raises(Exception):
         nat.check(st)
+
+
+@pytest.mark.parametrize("kind", ["cornell", "root_leaf"])
+def test_treelet_on_trees_smaller_than_the_treelet(kind):
+    """Passes of 2^19 path slots or more trace with the LDS copy of the top
+    of the tree.  A tree with fewer interior nodes than the treelet holds
+    (the Cornell box: 15 nodes) goes into it whole; a tree whose root is a
+    leaf (two triangles) gives an empty treelet and the traversal starts
+    from the root record."""
+    from roxy_dcc_amd.world import Camera, Diffuse, Emissive, IndexedMesh, Mat4, SceneObject, Shape, Sphere, World
+
+    _torch()
+    W, H = 768, 704    # 540,672 paths at one sample: above 2^19
+    if kind == "cornell":
+        res = helpers.render_pair("cornell", width=W, height=H, samples=1)
+    else:
+        w = World(background_color=(0.1, 0.1, 0.1), use_sky=False)
+        pos = np.array([[-2, 0, -2], [2, 0, -2], [2, 0, 2], [-2, 0, 2]], np.float64)
+        w.add_object(SceneObject("floor", [Shape(IndexedMesh(pos, np.int32([[0, 2, 1], [0, 3, 2]])),
+                                                 Diffuse((0.6, 0.5, 0.4)))], Mat4()))
+        w.add_object(SceneObject("light", [Shape(Sphere(0.3), Emissive((1.0, 1.0, 1.0), 10.0))],
+                                 Mat4.trs(translation=(0.0, 1.5, 0.0))))
+        w.add_camera(Camera(position=(0.0, 2.0, -4.0), forward=(0.0, -0.4, 1.0), fov=50, width=W, height=H))
+        res = helpers.render_pair(None, samples=1, world_tuple=(w, W, H, dict(SETTINGS)))
+        assert res["extract"].flat.triangles().triangle_count == 2
+    _assert_same(res, f"treelet {kind}")
