@@ -72,6 +72,7 @@ def lib():
         h = C.CDLL(str(LIB))
         h.orc_render_f32.restype = None
         h.orc_render_f64.restype = None
+        h.orc_render_f64_moments.restype = None
         _lib = h
     return _lib
 
@@ -244,9 +245,11 @@ def params(max_depth=4, use_sky=0, direct_light_mode=0, direct_light_max_depth=1
 class Accum:
     """Running means over a pixel set (all pixels, or a given list)."""
 
-    def __init__(self, n, precision="f32"):
+    def __init__(self, n, precision="f32", moments=False):
         self.precision = precision
         self.color = np.zeros((n, 3), np.float64 if precision == "f64" else np.float32)
+        # sum of squared (clamped) samples per pixel and channel, f64 only
+        self.sq = np.zeros((n, 3), np.float64) if moments else None
         self.normal = np.zeros((n, 3), np.float32)
         self.albedo = np.zeros((n, 3), np.float32)
         self.depth = np.zeros(n, np.float32)
@@ -263,9 +266,15 @@ def render(sc: OracleScene, cam: OrcCamera, prm: OrcParams, first, count, acc: A
         nl = len(plist)
     fn = lib().orc_render_f64 if acc.precision == "f64" else lib().orc_render_f32
     col = acc.color.ctypes.data_as(C.POINTER(C.c_double)) if acc.precision == "f64" else _f(acc.color)
-    fn(C.byref(sc.s), C.byref(cam), C.byref(prm), int(first), int(count), col, _f(acc.normal), _f(acc.albedo),
-       _f(acc.depth), C.byref(acc.counts), int(threads or threads_default()),
-       _i(plist) if plist is not None else None, nl)
+    args = [C.byref(sc.s), C.byref(cam), C.byref(prm), int(first), int(count), col, _f(acc.normal), _f(acc.albedo),
+            _f(acc.depth), C.byref(acc.counts), int(threads or threads_default()),
+            _i(plist) if plist is not None else None, nl]
+    if acc.sq is not None:
+        if acc.precision != "f64":
+            raise ValueError("second moments are kept by the fp64 instance only")
+        fn = lib().orc_render_f64_moments
+        args.append(acc.sq.ctypes.data_as(C.POINTER(C.c_double)))
+    fn(*args)
     acc.samples = first + count
     return acc
 

@@ -772,7 +772,7 @@ path_out<R> trace(const orc_scene& s, const orc_params& p, int pix, int frame, V
 template <class R>
 void render(const orc_scene& s, const orc_camera& cam, const orc_params& p, int first, int count, float* acc_c,
             float* acc_n, float* acc_a, float* acc_d, double* acc_c64, orc_counts* total, int threads,
-            const int32_t* pixel_list, int n_list) {
+            const int32_t* pixel_list, int n_list, double* acc_sq = nullptr) {
     int npix = cam.width * cam.height;
     int nwork = pixel_list ? n_list : npix;
     unsigned long long rays = 0, shadow = 0, pops = 0, tris = 0, prims = 0, spops = 0, stris = 0, sprims = 0;
@@ -803,6 +803,12 @@ void render(const orc_scene& s, const orc_camera& cam, const orc_params& p, int 
                 R lim = p.sample_clamp;
                 smp = v3(std::fmin(std::fmax(smp.x, (R)0), lim), std::fmin(std::fmax(smp.y, (R)0), lim),
                          std::fmin(std::fmax(smp.z, (R)0), lim));
+            }
+            if (acc_sq) {
+                // second moments of the accumulated samples (per-pixel variance)
+                acc_sq[3 * w] += (double)smp.x * (double)smp.x;
+                acc_sq[3 * w + 1] += (double)smp.y * (double)smp.y;
+                acc_sq[3 * w + 2] += (double)smp.z * (double)smp.z;
             }
             // rendering/taichi/trace.py:266-281
             if (frame == 0) {
@@ -960,6 +966,15 @@ void orc_render_f64(const orc_scene* s, const orc_camera* cam, const orc_params*
                     const int32_t* pixel_list, int n_list) {
     render<double>(*s, *cam, *p, first, count, nullptr, normal, albedo, depth, color, total, threads, pixel_list,
                    n_list);
+}
+
+// orc_render_f64 that also adds each sample's squared colour to sq (n x 3,
+// summed over calls): with the running mean, the per-pixel sample variance.
+void orc_render_f64_moments(const orc_scene* s, const orc_camera* cam, const orc_params* p, int first, int count,
+                            double* color, float* normal, float* albedo, float* depth, orc_counts* total,
+                            int threads, const int32_t* pixel_list, int n_list, double* sq) {
+    render<double>(*s, *cam, *p, first, count, nullptr, normal, albedo, depth, color, total, threads, pixel_list,
+                   n_list, sq);
 }
 
 }  // extern "C"

@@ -3,9 +3,10 @@
 For each configuration, a seeded subset of pixels is rendered by the CPU
 oracle's fp64 instance (the reference estimator of
 rendering/taichi/trace.py in double precision) at 4096 samples per pixel,
-and the per-pixel means are stored with the pixel indices:
+and the per-pixel means of the samples and of their squares are stored
+with the pixel indices:
 
-    tests/golden/rmse_<config>.npz  {pixels, color (n,3) f64, spp, seed, ...}
+    tests/golden/rmse_<config>.npz  {pixels, color (n,3) f64, color_sq (n,3) f64, spp, seed, ...}
 
 Run from the repository root:  python tests/golden/make_rmse_refs.py [config ...]
 The GPU tests compare renders of the same pixels against these means.
@@ -44,13 +45,13 @@ def make(name, n_pixels=None, spp=SPP):
     prm = orc.params(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
                      world._background_color, RENDER_SEED)
     pix = np.sort(np.random.default_rng(PIXEL_SEED).choice(W * H, size=n_pixels, replace=False)).astype(np.int32)
-    acc = orc.Accum(n_pixels, "f64")
+    acc = orc.Accum(n_pixels, "f64", moments=True)
     t0 = time.time()
     orc.render(osc, ocam, prm, 0, spp, acc, pixels=pix)
     dt = time.time() - t0
     out = ROOT / "tests" / "golden" / f"rmse_{name}.npz"
-    np.savez_compressed(out, pixels=pix, color=acc.color, spp=spp, render_seed=RENDER_SEED, width=W, height=H,
-                        max_depth=st["max_depth"])
+    np.savez_compressed(out, pixels=pix, color=acc.color, color_sq=acc.sq / spp, spp=spp, render_seed=RENDER_SEED,
+                        width=W, height=H, max_depth=st["max_depth"])
     print(f"{name}: {n_pixels} px x {spp} spp in {dt:.0f} s -> {out.name}")
 
 

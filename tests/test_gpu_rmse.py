@@ -58,3 +58,32 @@ def test_rmse_against_fp64_oracle(name):
         # the bound allows 0.2% of the mean radiance beyond the noise
         assert abs(bias) < 5 * se + 1e-4 + 2e-3 * float(np.mean(np.abs(want)))
     assert 1.5 < rm[16] / rm[64] < 2.6
+
+
+@pytest.mark.parametrize("name,spp", [("cornell", 64), ("mesh", 64), ("large", 16)])
+def test_pixels_within_three_sigma_of_fp64_oracle(name, spp):
+    """Per pixel and channel, the render against the fp64 oracle's 4096-spp
+    mean in units of the combined standard error sqrt(v / spp + v / 4096),
+    v the per-sample variance the oracle measured at that pixel (its second
+    moment minus the squared mean).  Pixels where all 4096 reference samples
+    were equal (v = 0: background, unlit or fully specular-to-background)
+    must match the reference value.  Gaussian errors would put 99.7% within
+    3 sigma.  Pixels whose variance comes from rare bright samples (implicit
+    light hits, clamped at 10) are not Gaussian at 64 samples: drawing one
+    such sample moves the mean by more than 3 sigma.  Measured: cornell
+    98.4%, mesh 98.9% (64 spp), large 99.4% (16 spp); the render equals the
+    fp32 oracle's, so these figures are deterministic.  The bound is 98%."""
+    ref = np.load(GOLDEN / f"rmse_{name}.npz")
+    pix, want = ref["pixels"], ref["color"]
+    var = np.maximum(ref["color_sq"] - want ** 2, 0.0)
+    got = _gpu_render(name, spp, seed=11)[pix].astype(np.float64)
+    sig = np.sqrt(var / spp + var / float(ref["spp"]))
+    spread = sig > 1e-9 * (1.0 + np.abs(want))
+    z = np.abs(got - want)[spread] / sig[spread]
+    frac = float((z <= 3).mean())
+    flat_ok = float((np.abs(got - want)[~spread] <= 1e-5 * (1.0 + np.abs(want[~spread]))).mean()) if (~spread).any() else 1.0
+    print(f"{name}: {frac * 100:.2f}% of {z.size} pixel channels within 3 sigma at {spp} spp "
+          f"(99th percentile {np.quantile(z, 0.99):.2f} sigma); {int((~spread).sum())} without variance, "
+          f"{flat_ok * 100:.2f}% of them equal", flush=True)
+    assert frac >= 0.98
+    assert flat_ok >= 0.99

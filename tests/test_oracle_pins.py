@@ -274,6 +274,29 @@ def test_fp64_oracle_agrees_with_fp32_in_expectation():
     assert np.abs(a32.color.mean() - a64.color.mean()) < 0.01 * max(a64.color.mean(), 1e-3)
 
 
+def test_second_moments_are_the_squares_of_the_samples():
+    """The fp64 instance's per-pixel second moments (the RMSE fixtures'
+    per-pixel variance) against the samples themselves: each sample rendered
+    alone gives sq = x^2 (x >= 0 after the clamp), and those samples must
+    reproduce both the running mean and the summed squares of one call."""
+    sc = _tri_scene(FLOOR, prims=_sphere_prims([0, 3, 0], 0.5, mat_type=3, emission=4.0, albedo=(1, 1, 1)),
+                    mats=[(0, (0.5, 0.5, 0.5), 0.0, 1.0, 0.0, -1)])
+    cam = _camera((0, 1.5, -2), (0, -0.5, 1), w=16, h=16)
+    p = orc.params(max_depth=4, seed=3)
+    n = 12
+    one = orc.Accum(256, "f64", moments=True)
+    orc.render(sc, cam, p, 0, n, one)
+    xs = []
+    for k in range(n):
+        a = orc.Accum(256, "f64", moments=True)
+        orc.render(sc, cam, p, k, 1, a)
+        xs.append(np.sqrt(a.sq))
+    xs = np.stack(xs)
+    np.testing.assert_allclose(xs.mean(0), one.color, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose((xs ** 2).sum(0), one.sq, rtol=1e-12, atol=1e-12)
+    assert (xs.std(0) > 0).any()
+
+
 def test_render_pixel_subset_matches_full_frame():
     sc = _tri_scene(FLOOR, prims=_sphere_prims([0, 3, 0], 0.5, mat_type=3, emission=4.0, albedo=(1, 1, 1)))
     cam = _camera((0, 1.5, -2), (0, -0.5, 1), w=16, h=16)
