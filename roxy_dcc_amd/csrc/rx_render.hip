@@ -316,7 +316,7 @@ __device__ __forceinline__ int wave_cap(int n, int block_size) {
 // has run dry.  seg / tried: the wave's current segment and how many it
 // has found empty (wave-uniform).
 #ifndef RX_REFILL_MIN
-#define RX_REFILL_MIN 40
+#define RX_REFILL_MIN 48
 #endif
 constexpr int REFILL_MIN = RX_REFILL_MIN;
 #ifndef RX_XCD_SEGMENTS
