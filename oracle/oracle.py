@@ -333,3 +333,19 @@ def camera_rays(cam: OrcCamera, frame):
     out = np.empty((cam.height * cam.width, 3), np.float32)
     lib().orc_camera_rays(C.byref(cam), int(frame), _f(out))
     return out
+
+
+SCATTER_KINDS = {"cosine": 0, "unit": 1, "metal": 2, "dielectric": 3}
+
+
+def scatter_dirs(kind, u1, u2=None, d=(0.0, 0.0, -1.0), n=(0.0, 0.0, 1.0), param=0.0):
+    """The oracle's scatter.py samplers on given uniforms (orc_scatter_dirs):
+    kind "cosine" (normal n), "unit", "metal" (incoming d, normal n, fuzz
+    param), "dielectric" (d, n, eta param, xi = u1).  Returns (count, 3)."""
+    u1 = np.ascontiguousarray(np.asarray(u1, np.float32))
+    u2 = np.ascontiguousarray(np.asarray(u1 if u2 is None else u2, np.float32))
+    out = np.empty((len(u1), 3), np.float32)
+    dv = (C.c_float * 3)(*map(float, d))
+    nv = (C.c_float * 3)(*map(float, n))
+    lib().orc_scatter_dirs(SCATTER_KINDS[kind], dv, nv, C.c_float(param), _f(u1), _f(u2), len(u1), _f(out))
+    return out

@@ -892,6 +892,28 @@ void orc_camera_rays(const orc_camera* cam, int frame, float* dirs) {
         }
 }
 
+// The direction samplers of rendering/taichi/scatter.py evaluated on given
+// uniforms (fp32), for distribution tests of the restatement:
+//   kind 0  cosine_hemisphere(n) / diffuse_scatter       (:14-38), u1 = r1, u2 = r2
+//   kind 1  random_unit_vector                            (:6-11),  u1, u2
+//   kind 2  metal_scatter(d, n, roughness = param)        (:50-54), u1, u2
+//   kind 3  dielectric_scatter(d, n, eta = param), xi = u1 (:71-81)
+// d and n are 3 floats; out is count x 3.
+void orc_scatter_dirs(int kind, const float* d, const float* n, float param, const float* u1, const float* u2,
+                      int count, float* out) {
+    V<float> dd = v3(d[0], d[1], d[2]), nn = v3(n[0], n[1], n[2]);
+    for (int i = 0; i < count; ++i) {
+        V<float> r = v3(0.0f, 0.0f, 0.0f);
+        if (kind == 0) r = cosine_dir<float>(nn, u1[i], u2[i]);
+        if (kind == 1) r = sphere_dir<float>(u1[i], u2[i]);
+        if (kind == 2) r = fuzzy_mirror<float>(dd, nn, param, u1[i], u2[i]);
+        if (kind == 3) r = glass_dir<float>(dd, nn, param, u1[i]);
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+}
+
 // Closest hit for n rays (SoA like the GPU entry point).
 void orc_closest_hit(const orc_scene* s, const float* rays, int n, float* t, int32_t* idx, int32_t* kind, float* u,
                      float* v, orc_counts* total, int threads) {

@@ -106,7 +106,7 @@ def test_uneven_jobs_and_oracle_parity():
         got = _color(j).reshape(-1, 3)
         same = np.all(got == acc.color, axis=1).mean()
         print(f"job {j.seed} {j.width}x{j.height}x{j.samples}: exact {same:.4f}", flush=True)
-        assert same >= 0.99
+        np.testing.assert_array_equal(got, acc.color)
     b.close()
 
 

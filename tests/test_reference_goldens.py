@@ -1,0 +1,132 @@
+"""The oracle against vectors written by the reference's own code (no GPU).
+
+tests/golden/bvh_builder.npz holds node arrays and leaf orders produced by
+rendering/taichi/bvh_builder.py's GPUBVHBuilder.build (both methods) and
+tests/golden/denoise.npz outputs of rendering/denoise.py's
+edge_aware_denoise; the scripts beside them say how they were made.  The
+numpy restatements in oracle/ must reproduce them bit for bit (trees) and
+within 1e-6 (denoise), and the package's host-side palette dedup must
+reproduce the reference's palette.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import bvh_oracle
+from oracle import denoise_oracle
+from tests.golden import make_bvh_builder_golden as gb
+from tests.golden import make_denoise_golden as gd
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIELDS = ("left", "right", "tri_start", "tri_count")
+
+
+def _bvh():
+    return np.load(GOLDEN / "bvh_builder.npz")
+
+
+def _tree_arrays(t):
+    return {"aabb_min": t.aabb_min, "aabb_max": t.aabb_max, "left": t.left, "right": t.right,
+            "tri_start": t.tri_start, "tri_count": t.tri_count, "order": t.order}
+
+
+def assert_tree_matches(arrays, g, prefix):
+    np.testing.assert_array_equal(arrays["order"], g[f"{prefix}_order"], err_msg=f"{prefix} order")
+    for f in FIELDS:
+        np.testing.assert_array_equal(arrays[f], g[f"{prefix}_{f}"], err_msg=f"{prefix} {f}")
+    for f in ("aabb_min", "aabb_max"):
+        np.testing.assert_array_equal(np.asarray(arrays[f], np.float32).view(np.uint32),
+                                      g[f"{prefix}_{f}"].view(np.uint32), err_msg=f"{prefix} {f}")
+
+
+BUILD = {"morton": bvh_oracle.build_morton, "median": bvh_oracle.build_median}
+
+
+@pytest.mark.parametrize("method", ["morton", "median"])
+@pytest.mark.parametrize("k", range(len(gb.CASES)))
+def test_bvh_oracle_matches_reference_builder(k, method):
+    g = _bvh()
+    n, seed = (int(x) for x in g["cases"][k])
+    kind = str(g["kinds"][k])
+    v0, v1, v2 = gb.mesh(n, seed, kind)
+    assert_tree_matches(_tree_arrays(BUILD[method](v0, v1, v2, 8)), g, f"case{k}_{method}")
+
+
+@pytest.mark.parametrize("method", ["morton", "median"])
+def test_bvh_oracle_matches_reference_builder_on_scene_meshes(method):
+    g = _bvh()
+    for name in ("cornell", "mesh"):
+        v0, v1, v2 = gb.scene_triangles(name)
+        t = BUILD[method](v0, v1, v2, 8)
+        assert [len(v0), t.n_nodes] == g[f"{name}_{method}_shape"].tolist(), name
+        if name == "cornell":
+            assert_tree_matches(_tree_arrays(t), g, f"{name}_{method}")
+        assert gb.digest(_tree_arrays(t)) == str(g[f"{name}_{method}_sha256"]), name
+
+
+def test_material_palette_matches_reference_builder():
+    """First-appearance dedup (bvh_builder.py:389-410) in the oracle and in
+    the package's host-side GPUBVHBuilder._build_material_indices."""
+    from roxy_dcc_amd.bvh import GPUBVHBuilder, TriangleBatch
+
+    g = _bvh()
+    v0, v1, v2, rows = gb.palette_case()
+    mats = [gb.PALETTE_ROWS[r] for r in rows]
+    pal, idx = bvh_oracle.material_palette([(m[0], m[1], m[2], m[3], m[4], -1) for m in mats])
+    assert [p[0] for p in pal] == g["palette_type"].tolist()
+    np.testing.assert_array_equal(np.float32([p[1] for p in pal]), g["palette_albedo"])
+    np.testing.assert_array_equal(np.float32([[p[2], p[3], p[4]] for p in pal]), g["palette_scalars"])
+    order = bvh_oracle.build_morton(v0, v1, v2, 8).order
+    np.testing.assert_array_equal(order, g["palette_order"])
+    np.testing.assert_array_equal(np.asarray(idx, np.int32)[order], g["palette_ordered_mat_idx"])
+
+    n = len(v0)
+    nrm = np.tile(np.float32([[0, 0, 1]]), (n, 1))
+    batch = TriangleBatch(v0, v1, v2, nrm, nrm, nrm, [m[0] for m in mats], [m[1] for m in mats],
+                          [m[2] for m in mats], [m[3] for m in mats], [m[4] for m in mats])
+    b = GPUBVHBuilder(device="cpu")
+    local = b._build_material_indices(batch)
+    assert [p["type"] for p in b.mat_palette] == g["palette_type"].tolist()
+    np.testing.assert_array_equal(np.float32([p["albedo"] for p in b.mat_palette]), g["palette_albedo"])
+    np.testing.assert_array_equal(local[order], g["palette_ordered_mat_idx"])
+
+
+def _denoise():
+    return np.load(GOLDEN / "denoise.npz")
+
+
+def denoise_inputs(g, c):
+    img, radius, guides, d3 = (int(x) for x in g["cases"][c])
+    sc, amount, sn, sa, sd = (float(x) for x in g["case_sigmas"][c])
+    color = g[f"img{img}_color"]
+    kw = dict(sigma_normal=sn, sigma_albedo=sa, sigma_depth=sd)
+    if guides & gd.GUIDE_NORMAL:
+        kw["normal"] = g[f"img{img}_normal"]
+    if guides & gd.GUIDE_ALBEDO:
+        kw["albedo"] = g[f"img{img}_albedo"]
+    if guides & gd.GUIDE_DEPTH:
+        depth = g[f"img{img}_depth"]
+        kw["depth"] = depth[..., None] if d3 else depth
+    return color, radius, sc, amount, kw
+
+
+def test_denoise_oracle_matches_reference():
+    g = _denoise()
+    worst = 0.0
+    for c in range(len(g["cases"])):
+        color, radius, sc, amount, kw = denoise_inputs(g, c)
+        out = np.asarray(denoise_oracle.denoise(color, radius, sc, amount, **kw), np.float32)
+        want = g[f"case{c}_out"]
+        assert out.shape == want.shape
+        err = float(np.abs(out - want).max())
+        worst = max(worst, err)
+        assert err <= 1e-6, (c, err)
+    print(f"denoise oracle vs reference: max |diff| {worst:.3g} over {len(g['cases'])} cases")
+
+
+def test_display_oracle_matches_reference():
+    g = _denoise()
+    np.testing.assert_array_equal(denoise_oracle.display(g["display_in"]), g["display_out"])
