@@ -409,9 +409,26 @@ int rx_batch_workspace_size(const rx_job* jobs, int n_jobs, size_t* bytes);
 int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace, size_t workspace_bytes,
                     int use_graph, rx_batch** out);
 int rx_batch_run(rx_batch* batch, int passes, rx_counters* d_counters, rx_profile* profile, void* stream);
+/* Node counts of the pass captured by rx_batch_run (kernels, memory fills,
+ * any other node), or -1 each before a capture.  No reference counterpart:
+ * lets a caller check that a replayed pass holds only kernel nodes. */
+int rx_batch_graph_nodes(const rx_batch* batch, int* kernels, int* memsets, int* others);
 /* Copies the job records (progress and ray counts) back; synchronises `stream`. */
 int rx_batch_read_jobs(rx_batch* batch, rx_job* jobs_out, void* stream);
 int rx_batch_destroy(rx_batch* batch);
+
+/* ---- ray-queue sort ------------------------------------------------------
+ * The stable LSD radix sort rx_render uses to reorder secondary ray queues
+ * (no reference counterpart: render_kernel traces pixels in place).  Sorts
+ * the first *d_count pairs (read on the device, clamped to cap) of
+ * (d_keys, d_vals) by the low key_bits of the keys; the sorted pairs end
+ * in (d_keys, d_vals), (d_keys2, d_vals2) are the second buffers (cap
+ * entries each).  Counts below min_n are left as they are.  Enqueues only
+ * kernels (no memory fills), so it can be captured in a HIP graph. */
+int rx_radix_sort_scratch_size(long long cap, size_t* bytes);
+int rx_radix_sort_pairs(uint32_t* d_keys, uint32_t* d_vals, uint32_t* d_keys2, uint32_t* d_vals2,
+                        const int32_t* d_count, long long cap, int min_n, int key_bits, void* d_scratch,
+                        size_t scratch_bytes, void* stream);
 
 /* ---- denoise: rendering/denoise.py:23-108 -------------------------------
  * Guided bilateral filter over a (2*radius+1)^2 window whose taps are

@@ -130,6 +130,9 @@ SIGNATURES = {
     "rx_batch_create": (C.c_int, [C.POINTER(RxJob), C.c_int, vp, C.c_size_t, C.c_int, C.POINTER(vp)]),
     "rx_batch_run": (C.c_int, [vp, C.c_int, vp, C.POINTER(RxProfile), vp]),
     "rx_batch_read_jobs": (C.c_int, [vp, C.POINTER(RxJob), vp]),
+    "rx_radix_sort_scratch_size": (C.c_int, [C.c_longlong, C.POINTER(C.c_size_t)]),
+    "rx_radix_sort_pairs": (C.c_int, [vp, vp, vp, vp, vp, C.c_longlong, C.c_int, C.c_int, vp, C.c_size_t, vp]),
+    "rx_batch_graph_nodes": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "rx_batch_destroy": (C.c_int, [vp]),
     "rx_denoise": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                              C.c_float, C.c_float, vp, vp]),

@@ -98,6 +98,13 @@ class Batch:
                                          nat.stream_ptr(stream)), "batch run")
         self.passes_run += passes
 
+    def graph_nodes(self):
+        """(kernel, memset, other) node counts of the captured pass, or
+        (-1, -1, -1) before a graph has been captured."""
+        k, m, o = C.c_int(), C.c_int(), C.c_int()
+        nat.check(nat.lib().rx_batch_graph_nodes(self._h, C.byref(k), C.byref(m), C.byref(o)), "batch graph nodes")
+        return k.value, m.value, o.value
+
     def read_jobs(self, stream=None):
         out = (nat.RxJob * len(self.jobs))()
         nat.check(nat.lib().rx_batch_read_jobs(self._h, out, nat.stream_ptr(stream)), "batch read")

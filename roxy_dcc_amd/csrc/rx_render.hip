@@ -24,8 +24,7 @@
 #include <new>
 
 #include "rx_launch.h"
-
-#include <rocprim/device/device_radix_sort.hpp>
+#include "rx_sort.h"
 
 namespace {
 
@@ -95,15 +94,15 @@ struct ws_view {
     int err_stride;
     // traversal stack spill area (entries past the LDS part)
     int2* spill;
-    // reordered copy of a ray queue for k_extend (single-job passes): the
-    // sort keys and slot indices (double buffers), rocPRIM's scratch;
-    // hit_slot (set per launch) maps a position of the traced order back
-    // to the queue slot its hit belongs to, ray_slot (if set) to the slot
-    // its ray is read from (null: position i is slot i)
+    // reordering of the ray queues for k_extend: the pixel-major copy of the
+    // primary queue, the sort keys and slot indices (double buffers) and
+    // the radix sort's digit counts (rx_sort.h); hit_slot (set per launch)
+    // maps a position of the traced order back to the queue slot its hit
+    // belongs to, ray_slot (if set) to the slot its ray is read from (null:
+    // position i is slot i)
     gv4f* q_sorted;
     uint32_t *sk_in, *sk_out, *si_in, *si_out;
-    void* sort_tmp;
-    size_t sort_bytes;
+    uint32_t* sort_scratch;
     const uint32_t* hit_slot;
     const uint32_t* ray_slot;
     // top of the tree for the traversal kernels' LDS (k_treelet): slot k's
@@ -115,15 +114,6 @@ struct ws_view {
     int32_t* tl_count;
     const rx_node* tl_nodes;
 };
-
-// Radix-sort scratch for reordering n ray slots (rocPRIM's own query).
-size_t ray_sort_bytes(long long n) {
-    size_t sb = 0;
-    rocprim::double_buffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
-    // sized for the widest key (32 bits); sorts of fewer bits need no more
-    if (rocprim::radix_sort_pairs(nullptr, sb, k, v, (unsigned int)(n > 0 ? n : 1), 0u, 32u) != hipSuccess) return 0;
-    return sb;
-}
 
 void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
     w.jobs = ar.take<rx_job>(n_jobs);
@@ -157,16 +147,14 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.ray_slot = nullptr;
     w.q_sorted = nullptr;
     w.sk_in = w.sk_out = w.si_in = w.si_out = nullptr;
-    w.sort_tmp = nullptr;
-    w.sort_bytes = 0;
+    w.sort_scratch = nullptr;
     if (sort) {
         w.q_sorted = ar.take<gv4f>(2 * P);
         w.sk_in = ar.take<uint32_t>(P);
         w.sk_out = ar.take<uint32_t>(P);
         w.si_in = ar.take<uint32_t>(P);
         w.si_out = ar.take<uint32_t>(P);
-        w.sort_bytes = ray_sort_bytes(P);
-        w.sort_tmp = ar.take_bytes(w.sort_bytes);
+        w.sort_scratch = ar.take<uint32_t>(sort_scratch_words(P));
     }
     w.n_jobs = n_jobs;
     w.P = P;
@@ -1657,35 +1645,25 @@ __device__ __forceinline__ uint32_t ray_key(f3 o, f3 d, const rx_bvh_header& h, 
     return (job << (9 + 3 * ob)) | (dq << (3 * ob)) | mo;
 }
 
-// Keys of the first n slots of queue b; slots past the queue's length
-// (read on the device) get the largest key, so a sort of the whole
-// capacity leaves them last.
-__global__ void k_ray_keys(ws_view w, int b, long long n, int ob, uint32_t* __restrict__ key,
-                           uint32_t* __restrict__ idx) {
+// Keys and the identity permutation of the slots of queue b, up to its
+// length (read on the device).
+__global__ void k_ray_keys(ws_view w, int b, int ob, uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
     const int count = w.counts[b];
-    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
-        uint32_t k = 0xffffffffu;
-        if (i < count) {
-            f3 o, d;
-            const int p = get_ray(w, b, (int)i, o, d);
-            const int jx = job_of_path(w, p);
-            const rx_scene& sc = w.jobs[jx].scene;
-            if (sc.n_tris > 0) {
-                k = ray_key(o, d, gload(sc.header), (uint32_t)jx, ob);
-            } else {
-                k = (uint32_t)jx << (9 + 3 * ob);
-            }
-        }
-        key[i] = k;
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < count; i += (long long)gridDim.x * BS) {
+        f3 o, d;
+        const int p = get_ray(w, b, (int)i, o, d);
+        const int jx = job_of_path(w, p);
+        const rx_scene& sc = w.jobs[jx].scene;
+        key[i] = sc.n_tris > 0 ? ray_key(o, d, gload(sc.header), (uint32_t)jx, ob) : (uint32_t)jx << (9 + 3 * ob);
         idx[i] = (uint32_t)i;
     }
 }
 
-// Queues shorter than this are traced as they are (when their length is
-// known on the host), and passes of fewer path slots than SORT_MIN_PATHS
-// do not read lengths back at all: for them the synchronisation per
-// bounce costs more than a sort saves (a loop of one-sample passes of
-// small frames would pay it thousands of times).
+// Queues shorter than this are traced in shade's order (the sort kernels
+// see the length on the device and leave the identity permutation), and
+// passes of fewer path slots than SORT_MIN_PATHS are not reordered at all:
+// for them the sort's launches cost more than they save (a loop of
+// one-sample passes of small frames would pay them thousands of times).
 constexpr int SORT_MIN_RAYS = 8192;
 constexpr long long SORT_MIN_PATHS = 1 << 20;
 
@@ -1705,16 +1683,14 @@ inline int key_origin_bits(int n_jobs) {
 }
 inline unsigned key_bits(int n_jobs) { return (unsigned)(key_job_bits(n_jobs) + 9 + 3 * key_origin_bits(n_jobs)); }
 
-// host_counts: the caller may synchronise (not under graph capture), so
-// each queue's length is read back and only that many rays are sorted;
-// otherwise the whole capacity is sorted, idle slots last.
+// Every queue length stays on the device: the sort reads it there, so a
+// pass never waits for the host and can be captured in a HIP graph.
 // Deep trees: 64-thread blocks, the stack mostly in LDS, no treelet.
 // Shallow trees: the large blocks that share one LDS treelet (measured on
 // the mesh: 2074 Mrays/s without it, 2168 with it; the 2M-triangle scene
 // is unchanged).
 template <bool DEEP>
-int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
-                 bool host_counts) {
+int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
     constexpr int EBS = DEEP ? BS_DEEP : BS_EXTEND;
     constexpr int CBS = DEEP ? BS_DEEP : BS_CONNECT;
     constexpr int TLN = DEEP ? 0 : TL_NODES;
@@ -1738,33 +1714,19 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
             we.q_ray[0] = w.q_sorted;
             we.hit_slot = w.si_in;
         }
-        if (d > 0 && pl.sort_rays && w.q_sorted && w.sort_bytes > 0) {
-            long long n = w.P;
-            if (host_counts && w.P < SORT_MIN_PATHS) {
-                n = 0;
-            } else if (host_counts) {
-                int cnt = 0;
-                RX_TRY_HIP(hipMemcpyAsync(&cnt, w.counts + qi, sizeof(int), hipMemcpyDeviceToHost, s));
-                RX_TRY_HIP(hipStreamSynchronize(s));
-                n = cnt >= SORT_MIN_RAYS ? cnt : 0;
-            }
-            if (n > 0) {
-                int gk = grid_for(n, BS, 16);
-                RX_STAGE(tm, 6, (k_ray_keys<<<gk, BS, 0, s>>>(w, qi, n, key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
-                rocprim::double_buffer<uint32_t> keys(w.sk_in, w.sk_out), slots(w.si_in, w.si_out);
-                size_t sb = w.sort_bytes;
-                int rs = tm.begin(6);
-                if (rs) return rs;
-                RX_TRY_HIP(rocprim::radix_sort_pairs(w.sort_tmp, sb, keys, slots, (unsigned int)n, 0u,
-                                                     key_bits(w.n_jobs), s));
-                rs = tm.end();
-                if (rs) return rs;
-                // k_extend reads each ray through the permutation: cheaper
-                // than gathering a sorted copy first (2.6 against 3.7 ms of
-                // reordering per 64-spp mesh render, same traversal time)
-                we.ray_slot = slots.current();
-                we.hit_slot = slots.current();
-            }
+        if (d > 0 && pl.sort_rays && w.q_sorted && w.P >= SORT_MIN_PATHS) {
+            RX_STAGE(tm, 6, (k_ray_keys<<<gp, BS, 0, s>>>(w, qi, key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
+            int rs = tm.begin(6);
+            if (rs) return rs;
+            RX_TRY_HIP(enqueue_ray_sort(w.sk_in, w.si_in, w.sk_out, w.si_out, w.counts + qi, w.P, SORT_MIN_RAYS,
+                                        key_bits(w.n_jobs), w.sort_scratch, s));
+            rs = tm.end();
+            if (rs) return rs;
+            // k_extend reads each ray through the permutation: cheaper
+            // than gathering a sorted copy first (2.6 against 3.7 ms of
+            // reordering per 64-spp mesh render, same traversal time)
+            we.ray_slot = w.si_in;
+            we.hit_slot = w.si_in;
         }
         if (pl.prim_prepass) {
             if (pl.count) {
@@ -1794,10 +1756,9 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     return RX_OK;
 }
 
-int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s,
-            bool host_counts) {
-    if (pl.deep) return enqueue_pass<true>(w, pl, ctr, tm, s, host_counts);
-    return enqueue_pass<false>(w, pl, ctr, tm, s, host_counts);
+int enqueue(const ws_view& w, const pass_plan& pl, rx_counters* ctr, stage_timer& tm, hipStream_t s) {
+    if (pl.deep) return enqueue_pass<true>(w, pl, ctr, tm, s);
+    return enqueue_pass<false>(w, pl, ctr, tm, s);
 }
 
 int check_job(const rx_job& j, int idx) {
@@ -1915,7 +1876,30 @@ struct rx_batch {
     hipGraphExec_t exec;
     bool captured;
     rx_counters* graph_ctr;
+    int graph_nodes[3];  // kernel, memset, other nodes of the captured pass
 };
+
+namespace {
+
+// Node types of a captured pass (kernels, memory fills, anything else).
+hipError_t count_graph_nodes(hipGraph_t g, int out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
+    if (e != hipSuccess || n == 0) return e;
+    hipGraphNode_t* nodes = (hipGraphNode_t*)malloc(sizeof(hipGraphNode_t) * n);
+    if (!nodes) return hipErrorOutOfMemory;
+    e = hipGraphGetNodes(g, nodes, &n);
+    for (size_t k = 0; e == hipSuccess && k < n; ++k) {
+        hipGraphNodeType t;
+        e = hipGraphNodeGetType(nodes[k], &t);
+        if (e == hipSuccess) out[t == hipGraphNodeTypeKernel ? 0 : (t == hipGraphNodeTypeMemset ? 1 : 2)]++;
+    }
+    free(nodes);
+    return e;
+}
+
+}  // namespace
 
 extern "C" int rx_render_workspace_size(int width, int height, int samples_per_pass, int bvh_depth, size_t* bytes) {
     RX_REQUIRE(bytes, RX_ERR_INVALID, "bytes is NULL");
@@ -1964,7 +1948,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     tm.s = s;
     int spp = params->samples_per_pass;
     for (int done = 0; done < n_samples; done += spp) {
-        st = enqueue(w, pl, d_counters, tm, s, true);
+        st = enqueue(w, pl, d_counters, tm, s);
         if (st) return st;
         if (profile) profile->n_passes += 1;
         if (tm.on) {
@@ -1972,6 +1956,25 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
             if (st) return st;
         }
     }
+    return RX_OK;
+}
+
+extern "C" int rx_radix_sort_scratch_size(long long cap, size_t* bytes) {
+    RX_REQUIRE(bytes && cap >= 0 && cap <= 0x7fffffffLL, RX_ERR_INVALID, "capacity must be in 0..2^31-1");
+    *bytes = sort_scratch_words(cap) * sizeof(uint32_t);
+    return RX_OK;
+}
+
+extern "C" int rx_radix_sort_pairs(uint32_t* d_keys, uint32_t* d_vals, uint32_t* d_keys2, uint32_t* d_vals2,
+                                   const int32_t* d_count, long long cap, int min_n, int key_bits, void* d_scratch,
+                                   size_t scratch_bytes, void* stream) {
+    RX_REQUIRE(d_keys && d_vals && d_keys2 && d_vals2 && d_count && d_scratch, RX_ERR_INVALID, "NULL argument");
+    RX_REQUIRE(cap >= 1 && cap <= 0x7fffffffLL, RX_ERR_INVALID, "capacity must be in 1..2^31-1");
+    RX_REQUIRE(key_bits >= 1 && key_bits <= 32, RX_ERR_INVALID, "key_bits must be in 1..32");
+    RX_REQUIRE(scratch_bytes >= sort_scratch_words(cap) * sizeof(uint32_t), RX_ERR_WORKSPACE,
+               "sort scratch too small");
+    RX_TRY_HIP(enqueue_ray_sort(d_keys, d_vals, d_keys2, d_vals2, d_count, cap, min_n < 0 ? 0 : min_n,
+                                (unsigned)key_bits, (uint32_t*)d_scratch, rx_stream(stream)));
     return RX_OK;
 }
 
@@ -1993,7 +1996,12 @@ extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace
     int32_t* poff = (int32_t*)malloc(sizeof(int32_t) * (n_jobs + 1));
     int32_t* qoff = (int32_t*)malloc(sizeof(int32_t) * (n_jobs + 1));
     rx_job* hj = (rx_job*)malloc(sizeof(rx_job) * n_jobs);
-    RX_REQUIRE(poff && qoff && hj, RX_ERR_INVALID, "out of host memory");
+    if (!poff || !qoff || !hj) {
+        free(poff);
+        free(qoff);
+        free(hj);
+        return rx::set_error(RX_ERR_INVALID, "out of host memory");
+    }
     long long P = 0, NP = 0;
     pass_plan pl;
     int st = plan_jobs(jobs, n_jobs, &P, &NP, &pl, poff, qoff);
@@ -2030,6 +2038,7 @@ extern "C" int rx_batch_create(const rx_job* jobs, int n_jobs, void* d_workspace
     b->exec = nullptr;
     b->captured = false;
     b->graph_ctr = nullptr;
+    b->graph_nodes[0] = b->graph_nodes[1] = b->graph_nodes[2] = -1;
     *out = b;
     return RX_OK;
 }
@@ -2049,17 +2058,20 @@ extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx
             RX_TRY_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             stage_timer off;
             off.on = false;
-            hipGraph_t g;
             hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
             int st = RX_OK;
-            if (e == hipSuccess) st = enqueue(b->w, b->plan, d_counters, off, cap, false);
-            hipError_t e2 = hipStreamEndCapture(cap, &g);
+            if (e == hipSuccess) st = enqueue(b->w, b->plan, d_counters, off, cap);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cap, &g) : e;
             (void)hipStreamDestroy(cap);
-            if (st) return st;
-            if (e != hipSuccess || e2 != hipSuccess)
+            if (st || e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (st) return st;
                 return rx::set_error(RX_ERR_HIP, "graph capture failed: %s / %s", hipGetErrorString(e),
                                      hipGetErrorString(e2));
-            hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            }
+            hipError_t e3 = count_graph_nodes(g, b->graph_nodes);
+            if (e3 == hipSuccess) e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e3 != hipSuccess)
                 return rx::set_error(RX_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e3));
@@ -2073,7 +2085,7 @@ extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx
     tm.on = prof;
     tm.s = s;
     for (int k = 0; k < passes; ++k) {
-        int st = enqueue(b->w, b->plan, d_counters, tm, s, true);
+        int st = enqueue(b->w, b->plan, d_counters, tm, s);
         if (st) return st;
         if (profile) profile->n_passes += 1;
         if (tm.on) {
@@ -2081,6 +2093,14 @@ extern "C" int rx_batch_run(rx_batch* b, int passes, rx_counters* d_counters, rx
             if (st) return st;
         }
     }
+    return RX_OK;
+}
+
+extern "C" int rx_batch_graph_nodes(const rx_batch* b, int* kernels, int* memsets, int* others) {
+    RX_REQUIRE(b && kernels && memsets && others, RX_ERR_INVALID, "NULL argument");
+    *kernels = b->graph_nodes[0];
+    *memsets = b->graph_nodes[1];
+    *others = b->graph_nodes[2];
     return RX_OK;
 }
 

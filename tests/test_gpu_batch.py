@@ -55,9 +55,9 @@ def test_batch_equals_alone_and_graph_equals_direct():
 
 def test_reordered_queues_in_batches_change_nothing():
     """Batches trace primaries pixel-major and sort their secondary queues
-    too (the whole queue capacity under a HIP graph, idle slots last; the
-    read-back length otherwise): every image equals that of a batch that
-    keeps each queue in path order."""
+    too (the sort reads each queue's length on the device, under a HIP
+    graph or not): every image equals that of a batch that keeps each queue
+    in path order."""
     import torch
 
     from roxy_dcc_amd import _native as nat
@@ -67,8 +67,8 @@ def test_reordered_queues_in_batches_change_nothing():
     imgs = {}
     for name, graph, flags in (("graph", True, 0), ("direct", False, 0),
                                ("kept", False, nat.RX_FLAG_KEEP_QUEUE_ORDER)):
-        # 3 jobs x 384^2 x 4 samples per pass: above the size whose
-        # queue lengths are read back
+        # 3 jobs x 384^2 x 4 samples per pass: above the pass size (2^20
+        # path slots) from which secondary queues are sorted
         jobs = _jobs(seeds, res_override=384, spp_pass=4)
         b = Batch(jobs, use_graph=graph, flags=flags)
         b.run()
@@ -144,10 +144,11 @@ def test_autotuned_shapes_leave_images_unchanged():
 
 
 def test_every_graph_replay_equals_the_eager_pass():
-    """The captured pass (which holds rocPRIM's sort and the memory fills
-    that library records) is replayed one pass at a time, with eager device
-    work and synchronisations between the replays; after every replay each
-    image and the ray count equal those of the same batch run eagerly."""
+    """The captured pass (which holds the ray-queue radix sort) is replayed
+    one pass at a time, with eager device work and synchronisations between
+    the replays; after every replay each image and the ray count equal
+    those of the same batch run eagerly.  The captured graph holds kernel
+    nodes only: no memory-fill node."""
     import torch
 
     from roxy_dcc_amd.batch import Batch
@@ -170,5 +171,9 @@ def test_every_graph_replay_equals_the_eager_pass():
             assert np.array_equal(_color(gj), _color(ej)), f"replay {k}: job {gj.seed} differs"
         assert g.rays() == e.rays(), f"replay {k}: ray counts differ"
         print(f"replay {k}: {len(seeds)} images equal, {g.rays()} rays", flush=True)
+    kernels, memsets, others = g.graph_nodes()
+    print(f"captured pass: {kernels} kernel nodes, {memsets} memset nodes, {others} other nodes", flush=True)
+    assert kernels > 0 and memsets == 0 and others == 0
+    assert e.graph_nodes() == (-1, -1, -1)
     g.close()
     e.close()
