@@ -1,20 +1,28 @@
 """Benchmark: path-tracing throughput on one MI355X.
 
-A step renders the mesh configuration once: a seeded displaced icosphere
-(69,620 triangles) in the Cornell enclosure with one sphere light, 512x512,
-64 samples per pixel, depth 8, next-event estimation, then 5 a-trous
-passes of the edge-aware denoise.  Scene flattening and the GPU BVH build
-happen once before the timed region; inputs are resident in HBM.
+A step renders the large configuration once: 500 seeded displaced UV
+spheres (2,000,000 triangles), 64 analytic spheres and three sphere lights
+in a 100^3 box, 1920x1080, 16 samples per pixel, depth 6, next-event
+estimation (--config picks another workload).  Scene flattening and the
+GPU BVH build happen once before the timed region; inputs are resident in
+HBM.
 
 Prints one JSON line:
   value        Mrays/s over the timed steps (rays counted as the reference
                counts them: one per bounce-loop iteration plus one per
                shadow ray cast)
-  roofline     the closest-hit (extend) kernel: algorithmic bytes
-               64*node pops + 36*triangle tests + 16*primitive tests per
-               launch over its event-timed average duration, against 8 TB/s
+  roofline     the closest-hit (extend) kernel: HBM bytes per launch
+               (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, committed under
+               profiles/) over its average launch time measured with HIP
+               events in the timed region, against 8 TB/s; the 64/36/16-byte
+               record rate beside it as algorithmic_gbs
+  rmse         against the fp64 oracle at 4096 spp on the fixture's pixels
+  configs      the other single-scene workloads (Cornell, mesh), the same way
+  batching     256 uneven scenes in one batch against one request at a time
+  serving      Poisson arrivals: the continuous-batching server against the
+               one-request-at-a-time loop, p50/p99 latency
   cpu_baseline the CPU oracle (fp32 restatement of the reference kernel)
-               on a bounded pixel subset of the same workload
+               on a bounded pixel subset of the headline workload
 """
 from __future__ import annotations
 
@@ -39,16 +47,27 @@ NODE_B, TRI_B, PRIM_B = 64, 36, 16
 def _load_traffic(name, samples_per_pass):
     """HBM bytes per extend launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel of known size), if recorded for the
-    same workload and pass size (a launch's traffic grows with the pass)."""
+    same workload and pass size (a launch's traffic grows with the pass),
+    and the counters that say what binds the kernel."""
     p = ROOT / "profiles" / f"traffic_{name}.json"
     if p.exists():
         try:
             t = json.loads(p.read_text())
         except Exception:
-            return None
+            return None, None
         if t.get("samples_per_pass") == samples_per_pass:
-            return t.get("bytes_per_launch")
-    return None
+            return t.get("bytes_per_launch"), t.get("binding")
+    return None, None
+
+
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(name, target_seconds=15.0, threads=None):
@@ -67,7 +86,7 @@ def cpu_baseline(name, target_seconds=15.0, threads=None):
     mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
     prm = orc.params(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
                      world._background_color, 0)
-    threads = threads or os.cpu_count() or 1
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
     rng = np.random.default_rng(0)
     n_pix = 256
     spp = st["samples"]
@@ -82,60 +101,36 @@ def cpu_baseline(name, target_seconds=15.0, threads=None):
         pix_done += n_pix
         n_pix = min(n_pix * 2, 65536)
     return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": f"{pix_done} random pixels x {spp} spp of the {W}x{H} {name} frame, fp32 oracle, "
                       f"{elapsed:.1f} s"}
 
 
-def _render_measure(R, nat, torch, world, W, H, st, sc, steps, flags_time, per_pass=None, extra_flags=0):
-    """Time `steps` full renders of one scene (profiled), plus one counted render."""
-    spp = st["samples"]
-    per_pass = per_pass or R.default_samples_per_pass(W, H, spp)
-    integ = R.Integrator(sc, W, H, per_pass)
-    frame = R.DeviceFrame(W, H)
-    cam = R.camera_struct(world.active_camera, W, H)
-    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
-
-    def render(flags):
-        prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
-                              world._background_color, 0, per_pass, flags | extra_flags)
-        done = 0
-        while done < spp:
-            n = min(per_pass, spp - done)
-            integ.render(frame, cam, prm, done, n)
-            done += n
-
-    render(0)
-    integ.reset_counts()
-    render(nat.RX_FLAG_COUNT_TRAFFIC)
-    torch.cuda.synchronize()
-    work = integ.counts()
-    integ.reset_counts()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        render(flags_time)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    return integ, frame, work, dt, per_pass
-
-
-def _roofline(work, prof, steps, traffic):
-    """k_extend's algorithmic bytes per launch over its average launch time.
-    When the primitive scan ran as its own pre-pass (k_prim_scan, scenes with
-    many analytic primitives) its tests are not k_extend's work."""
+def _roofline(work, prof, steps, traffic, pmc=None):
+    """k_extend's roofline: the HBM bytes one launch moves (rocprofv3
+    FETCH_SIZE x2 + WRITE_SIZE, from the committed profile of this workload
+    at this pass size) over its average launch time measured live with HIP
+    events, against MI355X's 8 TB/s.  The 64/36/16-byte count of node,
+    triangle and primitive records the traversal consumes (most served by
+    L2 and the Infinity Cache) is reported beside it as
+    ``algorithmic_gbs``, not as a fraction of HBM.  When the primitive scan
+    ran as its own pre-pass (k_prim_scan) its tests are not k_extend's
+    work."""
     alg = NODE_B * work["node_pops"] + TRI_B * work["tri_tests"]
     if prof.n_prim_scan == 0:
         alg += PRIM_B * work["prim_tests"]
     launches = prof.n_extend / steps
-    bytes_per_launch = alg / max(launches, 1)
+    alg_per_launch = alg / max(launches, 1)
     ms = prof.ms_extend / max(prof.n_extend, 1)
-    achieved = bytes_per_launch / (ms * 1e-3) / 1e9
-    # measured HBM bytes over the same launch time: below `achieved` when
-    # node and triangle fetches hit in L2 / the Infinity Cache
-    hbm = traffic / (ms * 1e-3) / 1e9 if traffic else None
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_extend (closest-hit traversal)",
-            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "launches_per_step": launches,
-            "hbm_traffic_gbs": hbm}
+    achieved = traffic / (ms * 1e-3) / 1e9 if traffic else None
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+           "kernel": "k_extend (closest-hit traversal)", "avg_launch_ms": ms, "launches_per_step": launches,
+           "algorithmic_bytes_per_launch": alg_per_launch,
+           "algorithmic_gbs": alg_per_launch / (ms * 1e-3) / 1e9}
+    if pmc:
+        out["binding"] = pmc
+    return out
 
 
 def _stages(prof, steps):
@@ -143,23 +138,14 @@ def _stages(prof, steps):
             ("generate", "prim_scan", "reorder", "extend", "shade", "connect", "accumulate")}
 
 
-def large_scene_extras(steps=2, extra_flags=0):
-    """The 2M-triangle 1080p scene: GPU BVH build time, traversal roofline, Mrays/s."""
+def bvh_build_times(ex, reps=5):
+    """GPU BVH builds over the scene's resident triangles: the Morton build
+    (bvh_builder.py:412-486) and core/bvh.py's object-median rule."""
     import torch
 
     from roxy_dcc_amd import _native as nat
-    from roxy_dcc_amd import renderer as R
-    from roxy_dcc_amd import scenes
     from roxy_dcc_amd.bvh import GPUBVHBuilder
-    from roxy_dcc_amd.extract import extract_scene
 
-    world, W, H, st = scenes.large_scene()
-    t0 = time.perf_counter()
-    ex = extract_scene(world)
-    torch.cuda.synchronize()
-    setup = time.perf_counter() - t0
-    sc = ex.scene
-    # GPU BVH build alone, triangles already resident
     b = GPUBVHBuilder()
     tb = ex.flat.triangles()
     b.build([tb])
@@ -170,7 +156,6 @@ def large_scene_extras(steps=2, extra_flags=0):
     work_buf = torch.empty(ws.value, dtype=torch.uint8, device="cuda")
     ref = b._ref_struct()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 5
     ev0.record()
     for _ in range(reps):
         nat.check(nat.lib().rx_bvh_build_morton(b.d["v0"].data_ptr(), b.d["v1"].data_ptr(), b.d["v2"].data_ptr(), n,
@@ -180,8 +165,6 @@ def large_scene_extras(steps=2, extra_flags=0):
     ev1.record()
     torch.cuda.synchronize()
     build_ms = ev0.elapsed_time(ev1) / reps
-    # core/bvh.py's object-median rule over the same resident triangles
-    # (fp64 vertices; host topology + per-level reduce and sorts)
     om = GPUBVHBuilder()
     om._reset_for_build(tb)
     om._build_object_median_gpu()
@@ -198,22 +181,91 @@ def large_scene_extras(steps=2, extra_flags=0):
             om_buf.data_ptr(), om_ws.value, C.byref(om_ref), om.d["nodes"].data_ptr(), om.d["header"].data_ptr(),
             om.d["order"].data_ptr(), nat.stream_ptr()))
     om_ms = (time.perf_counter() - t0) * 1e3 / reps
-    om_shape = (om.node_count, om.depth)
-    del om, v64, om_buf
-    integ, frame, work, dt, per_pass = _render_measure(R, nat, torch, world, W, H, st, sc, steps, nat.RX_FLAG_PROFILE,
-                                                       extra_flags=extra_flags)
+    return {"triangles": n, "morton_nodes": b.node_count, "morton_depth": b.depth, "gpu_morton_build_ms": build_ms,
+            "gpu_object_median_build_ms": om_ms, "object_median_nodes_depth": (om.node_count, om.depth)}
+
+
+def _rmse(name, frame):
+    """RMSE of the linear accumulator against the fp64 oracle's 4096-spp means
+    on the fixture's pixels (tests/golden/rmse_<name>.npz)."""
+    p = ROOT / "tests" / "golden" / f"rmse_{name}.npz"
+    if not p.exists():
+        return None
+    ref = np.load(p)
+    got = frame.color.cpu().numpy().reshape(-1, 3)[ref["pixels"]].astype(np.float64)
+    err = got - ref["color"]
+    return {"rmse": float(np.sqrt(np.mean(err ** 2))), "mean_error": float(err.mean()),
+            "pixels": int(len(ref["pixels"])), "reference_spp": int(ref["spp"])}
+
+
+def config_measure(name, steps, warmup=1, extra_flags=0):
+    """One single-scene workload: `steps` timed renders (stage times from HIP
+    events), the traversal work of one counted render, RMSE."""
+    import torch
+
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd import scenes
+    from roxy_dcc_amd.denoise import edge_aware_denoise, linear_to_display
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, W, H, st = scenes.CONFIGS[name]()
+    t0 = time.perf_counter()
+    ex = extract_scene(world)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    sc = ex.scene
+    spp = st["samples"]
+    per_pass = R.default_samples_per_pass(W, H, spp)
+    integ = R.Integrator(sc, W, H, per_pass)
+    frame = R.DeviceFrame(W, H)
+    cam = R.camera_struct(world.active_camera, W, H)
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    denoise_passes = st.get("denoise_passes", 0)
+
+    def step(flags):
+        prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                              world._background_color, 0, per_pass, flags | extra_flags)
+        done = 0
+        while done < spp:
+            n = min(per_pass, spp - done)
+            integ.render(frame, cam, prm, done, n)
+            done += n
+        if denoise_passes:
+            x = frame.color
+            for k in range(denoise_passes):
+                x = edge_aware_denoise(x, 1, 0.08, 0.8, normal=frame.normal, albedo=frame.albedo,
+                                       depth=frame.depth, dilation=1 << k)
+            linear_to_display(x)
+
+    for _ in range(warmup):
+        step(0)
+    integ.reset_counts()
+    step(nat.RX_FLAG_COUNT_TRAFFIC)
+    torch.cuda.synchronize()
+    work = integ.counts()
+    integ.reset_counts()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(nat.RX_FLAG_PROFILE)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
     counts = integ.counts()
-    return {"workload": f"large: {W}x{H}, {st['samples']} spp, depth {st['max_depth']}, {sc.n_tris} triangles, "
-                        f"{sc.n_prims} primitives, {sc.n_lights} lights",
-            "bvh_nodes": sc.builder.node_count, "bvh_depth": sc.builder.depth,
-            "gpu_bvh_build_ms": build_ms, "scene_setup_s": setup,
-            "gpu_object_median_build_ms": om_ms, "object_median_nodes_depth": om_shape,
-            "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
-            "samples_per_s": W * H * st["samples"] * steps / dt,
-            "stage_ms_per_render": _stages(integ.profile, steps),
-            "samples_per_pass": per_pass,
-            "roofline": _roofline(work, integ.profile, steps, _load_traffic("large", per_pass)),
-            "traversal_work_per_render": work}
+    traffic, pmc = _load_traffic(name, per_pass)
+    return {
+        "workload": f"{name}: {W}x{H}, {spp} spp, depth {st['max_depth']}, NEE {st['direct_light_mode']}, "
+                    f"{sc.n_tris} triangles, {sc.n_prims} primitives, {sc.n_lights} lights, "
+                    f"{denoise_passes} a-trous denoise passes",
+        "steps": steps, "dt": dt, "rays": counts["rays"],
+        "Mrays_per_s": counts["rays"] / dt / 1e6, "ms_per_render": dt * 1e3 / steps,
+        "renders_per_s": steps / dt, "samples_per_s": W * H * spp * steps / dt,
+        "rays_per_render": counts["rays"] // steps, "samples_per_pass": per_pass,
+        "stage_ms_per_render": _stages(integ.profile, steps), "scene_setup_s": setup,
+        "roofline": _roofline(work, integ.profile, steps, traffic, pmc),
+        "traversal_work_per_render": work, "rmse": _rmse(name, frame),
+        "_ex": ex,
+    }
 
 
 def _plain_render(R, torch, world, W, H, st, sc, seed, per_sample=False):
@@ -322,8 +374,10 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
 
     def load(key):
         kind, sd = key
-        world, *_ = scenes.mesh_scene(seed=sd) if kind == "mesh" else scenes.large_scene(seed=sd)
-        return extract_scene(world).scene, world.active_camera
+        world, _, _, st = scenes.mesh_scene(seed=sd) if kind == "mesh" else scenes.large_scene(seed=sd)
+        return extract_scene(world).scene, world.active_camera, {
+            "use_sky": bool(world._use_sky), "background": tuple(world._background_color),
+            "direct_light_mode": st["direct_light_mode"], "direct_light_max_depth": st["direct_light_max_depth"]}
 
     rng = np.random.default_rng(seed)
     keys = [("mesh", 20 + k) for k in range(6)] + [("large", 30 + k) for k in range(2)]
@@ -367,7 +421,7 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
         done = serial_loop(cache, requests(arr), paths_per_pass=1)
         row["serial_per_sample"] = latency_stats(done, time.perf_counter() - t0)
         same = 0
-        for policy in ("fair", "fifo", "srpt"):
+        for policy in ("deadline", "srpt", "fifo"):
             server = RenderServer(cache, max_jobs=32, policy=policy)
             t0 = time.perf_counter()
             done = server.run(requests(arr))
@@ -388,13 +442,13 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="mesh", choices=["cornell", "mesh", "large"])
+    ap.add_argument("--config", default="large", choices=["cornell", "mesh", "large"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--samples-per-pass", type=int, default=None)
-    ap.add_argument("--no-large", action="store_true", help="skip the 2M-triangle 1080p scene extras")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other single-scene workloads")
+    ap.add_argument("--no-bvh", action="store_true", help="skip the BVH build timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batching comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) comparison")
     ap.add_argument("--prim-prepass", action="store_true",
@@ -420,91 +474,46 @@ def main():
         return
 
     from roxy_dcc_amd import _native as nat
-    from roxy_dcc_amd import renderer as R
-    from roxy_dcc_amd import scenes
-    from roxy_dcc_amd.denoise import edge_aware_denoise, linear_to_display
-    from roxy_dcc_amd.extract import extract_scene
 
     nat.require_gpu()
-    world, W, H, st = scenes.CONFIGS[args.config]()
-    t0 = time.perf_counter()
-    ex = extract_scene(world)
-    torch.cuda.synchronize()
-    t_extract = time.perf_counter() - t0
-    sc = ex.scene
-    spp = st["samples"]
-    per_pass = args.samples_per_pass or R.default_samples_per_pass(W, H, spp)
-    integ = R.Integrator(sc, W, H, per_pass)
-    frame = R.DeviceFrame(W, H)
-    cam = R.camera_struct(world.active_camera, W, H)
-    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
-    denoise_passes = st.get("denoise_passes", 0)
     extra_flags = nat.RX_FLAG_PRIM_PREPASS if args.prim_prepass else 0
-
-    def step(flags):
-        prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
-                              world._background_color, 0, per_pass, flags | extra_flags)
-        done = 0
-        while done < spp:
-            n = min(per_pass, spp - done)
-            integ.render(frame, cam, prm, done, n)
-            done += n
-        if denoise_passes:
-            x = frame.color
-            for k in range(denoise_passes):
-                x = edge_aware_denoise(x, 1, 0.08, 0.8, normal=frame.normal, albedo=frame.albedo,
-                                       depth=frame.depth, dilation=1 << k)
-            linear_to_display(x)
-
-    for _ in range(args.warmup):
-        step(0)
-    # traversal work of one step (deterministic: same counts every step)
-    integ.reset_counts()
-    step(nat.RX_FLAG_COUNT_TRAFFIC)
-    torch.cuda.synchronize()
-    work = integ.counts()
-    integ.reset_counts()
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(nat.RX_FLAG_PROFILE)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    counts = integ.counts()
-    prof = integ.profile
-    rays = counts["rays"]
-    ms_step = dt * 1e3 / args.steps
-
-    roofline = _roofline(work, prof, args.steps, _load_traffic(args.config, per_pass))
-
+    head = config_measure(args.config, args.steps, args.warmup, extra_flags)
+    ex = head.pop("_ex")
     out = {
         "metric": "Mrays/s",
-        "value": rays / dt / 1e6,
+        "value": head["Mrays_per_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_render"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {W}x{H}, {spp} spp, depth {st['max_depth']}, NEE one, "
-                               f"{sc.n_tris} triangles, {sc.n_prims} primitives, {denoise_passes} a-trous denoise "
-                               "passes",
-                   "samples_per_pass": per_pass, "parallelism": "single GPU"},
-        "renders_per_s": args.steps / dt,
-        "samples_per_s": W * H * spp * args.steps / dt,
-        "rays_per_step": rays // args.steps,
-        "stage_ms_per_step": _stages(prof, args.steps),
-        "scene_setup_s": t_extract,
-        "roofline": roofline,
-        "traversal_work_per_step": work,
+        "config": {"workload": head["workload"], "samples_per_pass": head["samples_per_pass"],
+                   "parallelism": "single GPU"},
+        "renders_per_s": head["renders_per_s"],
+        "samples_per_s": head["samples_per_s"],
+        "rays_per_step": head["rays_per_render"],
+        "stage_ms_per_step": head["stage_ms_per_render"],
+        "scene_setup_s": head["scene_setup_s"],
+        "roofline": head["roofline"],
+        "rmse": head["rmse"],
+        "traversal_work_per_step": head["traversal_work_per_render"],
     }
-    if not args.no_large and args.config != "large":
-        out["large_scene"] = large_scene_extras(extra_flags=extra_flags)
+    if not args.no_bvh:
+        out["bvh_build"] = bvh_build_times(ex)
+    del ex
+    if not args.no_configs:
+        extra = {}
+        for name in ("cornell", "mesh", "large"):
+            if name != args.config:
+                r = config_measure(name, 3 if name != "cornell" else 10, 1, extra_flags)
+                r.pop("_ex")
+                extra[name] = r
+        out["configs"] = extra
     if not args.no_batch:
         out["batching"] = batching_bench()
     if not args.no_serve:
@@ -512,6 +521,12 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+    # release every device buffer before the interpreter's own teardown
+    import gc
+
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
 
 
 if __name__ == "__main__":

@@ -4,10 +4,11 @@ Requests (scene, resolution, samples) arrive over time.  The server keeps
 up to ``max_scenes`` scene BVHs resident (least recently used evicted,
 never one an active job needs), admits waiting requests in arrival order
 while job slots and the frame-memory budget allow, and every iteration
-traces one wavefront pass of *all* active jobs together (``Batch``).  The
-pass's path budget is shared out by water-filling (``share_pass``), so the
-device stays full while small jobs finish in a pass or two and large ones
-take what is left.  Finished jobs are retired and their frames read back;
+traces one wavefront pass of the active jobs together (``Batch``).  The
+pass's path budget is shared out by ``share_pass`` (shortest remaining work
+first, guarded by first-come-first-served deadlines), so the device stays
+full while small jobs finish in a pass or two and large ones take what is
+left.  Finished jobs are retired and their frames read back;
 new arrivals join at the next iteration, so their work overlaps with jobs
 already on the GPU.  A job's image is the same alone and in any mix
 (sampling is keyed by job seed, pixel, sample and bounce; accumulation is
@@ -42,6 +43,12 @@ class Request:
     # check_interval) of the adaptive stop (rendering/ray_tracer.py:531-541),
     # `samples` then being the cap
     adaptive: tuple = None
+    # render settings; None takes the scene's own (the world's sky or
+    # background and NEE settings, when the cache's loader gives them)
+    use_sky: bool = None
+    background: tuple = None
+    direct_light_mode: str = None
+    direct_light_max_depth: int = None
     # filled in by the server
     admitted: float = None
     finished: float = None
@@ -55,10 +62,14 @@ class Request:
 
 
 class SceneCache:
-    """Resident DeviceScenes keyed by scene key, LRU under a byte budget."""
+    """Resident DeviceScenes keyed by scene key, LRU under a byte budget.
+    The loader maps a key to (DeviceScene, camera) or (DeviceScene, camera,
+    settings), settings a dict of the scene's own render settings (use_sky,
+    background, direct_light_mode, direct_light_max_depth); ``get`` returns
+    the triple."""
 
     def __init__(self, loader, max_scenes=32, max_bytes=64 << 30):
-        self.loader = loader              # key -> (DeviceScene, camera)
+        self.loader = loader
         self.max_scenes = max_scenes
         self.max_bytes = max_bytes
         self.entries = collections.OrderedDict()
@@ -66,17 +77,18 @@ class SceneCache:
         self.evictions = 0
 
     def bytes(self):
-        return sum(s.hbm_bytes() for s, _ in self.entries.values())
+        return sum(e[0].hbm_bytes() for e in self.entries.values())
 
     def get(self, key, pinned=()):
         if key in self.entries:
             self.entries.move_to_end(key)
             return self.entries[key]
-        scene, cam = self.loader(key)
+        got = tuple(self.loader(key))
+        entry = got if len(got) == 3 else (got[0], got[1], {})
         self.loads += 1
-        self.entries[key] = (scene, cam)
+        self.entries[key] = entry
         self._evict(pinned=set(pinned) | {key})
-        return scene, cam
+        return entry
 
     def _evict(self, pinned):
         while (len(self.entries) > self.max_scenes or self.bytes() > self.max_bytes):
@@ -87,22 +99,73 @@ class SceneCache:
             self.evictions += 1
 
 
-def share_pass(demands, budget, policy="fair"):
+def deadline_order(work, slack):
+    """Shortest remaining work first, guarded by first-come-first-served
+    deadlines.  ``work`` is each job's remaining paths in arrival order.
+    Serving the jobs one after another in arrival order would finish job k
+    after prefix(k) = work[0] + ... + work[k] paths: that, plus ``slack``
+    paths, is job k's deadline.  The order starts as shortest-first; while
+    some job would finish past its deadline, the first such job moves ahead
+    of the first job with a later deadline.  Arrival order meets every
+    deadline, so if the moves have not converged after n^2 of them, the
+    order falls back to it."""
+    n = len(work)
+    deadline = []
+    run = 0
+    for w in work:
+        run += w
+        deadline.append(run + slack)
+    order = sorted(range(n), key=lambda k: (work[k], k))
+    for _ in range(n * n + 1):
+        t, late = 0, None
+        for pos, k in enumerate(order):
+            t += work[k]
+            if t > deadline[k]:
+                late = pos
+                break
+        if late is None:
+            return order
+        k = order.pop(late)
+        ins = next((i for i, j in enumerate(order) if deadline[j] > deadline[k]), late)
+        order.insert(ins, k)
+    return list(range(n))
+
+
+def share_pass(demands, budget, policy="deadline"):
     """Split of a pass's path budget: `demands` are (pixels, samples_left)
     per job in arrival order; returns samples per pass for each (whole
-    samples, at least one, at most what the job still needs).
+    samples, at most what the job still needs; 0 = not in this pass).
 
+    "deadline" (the default): jobs take the budget in deadline_order
+    (shortest remaining work first unless that would make some job finish
+    later than first-come-first-served would, with one pass of slack), each
+    as many samples as it can use; jobs whose next sample no longer fits
+    wait for a later pass, and the first job in the order always advances.
+    Low median latency like shortest-first, a tail no worse than the
+    arrival-order loop's.
     "fair": water-filling, smallest remaining work first, each job taking
     at most an equal share of what is left, so small jobs finish in one
     pass and the budget they leave goes to the larger ones (processor
-    sharing: low median latency).
+    sharing).
     "fifo": jobs take the budget in arrival order, as much as each can use,
     and every other job still advances one sample per pass (first come
-    first served with backfill: bounds the longest wait).
+    first served with backfill).
     "srpt": as "fifo", but in order of remaining work (pixels x samples
     left), smallest first: shortest remaining processing time with
-    backfill, the order that minimises mean latency."""
+    backfill; it starves the longest jobs under load."""
     n = len(demands)
+    if policy == "deadline":
+        out = [0] * n
+        order = deadline_order([px * need for px, need in demands], budget)
+        left = budget
+        for rank, k in enumerate(order):
+            px, need = demands[k]
+            take = min(need, left // max(1, px))
+            if rank == 0:
+                take = max(1, take)
+            out[k] = take
+            left -= take * px
+        return out
     out = [1] * n
     if policy in ("fifo", "srpt"):
         order = range(n) if policy == "fifo" else sorted(range(n), key=lambda k: demands[k][0] * demands[k][1])
@@ -126,6 +189,21 @@ def share_pass(demands, budget, policy="fair"):
     return out
 
 
+def request_job(req: Request, scene, cam, opts, samples_per_pass):
+    """The RenderJob of a request: its own settings where given, else the
+    scene's (``opts`` from the cache's loader), else RenderJob's defaults."""
+
+    def pick(name, default):
+        v = getattr(req, name)
+        return opts.get(name, default) if v is None else v
+
+    return RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
+                     samples_per_pass=samples_per_pass, adaptive=req.adaptive, tag=req,
+                     use_sky=bool(pick("use_sky", False)), background=tuple(pick("background", (0.0, 0.0, 0.0))),
+                     direct_light_mode=pick("direct_light_mode", "one"),
+                     direct_light_max_depth=int(pick("direct_light_max_depth", 1)))
+
+
 class RenderServer:
     """Continuous batching: up to ``max_jobs`` requests share every pass.
 
@@ -133,14 +211,14 @@ class RenderServer:
     The pass's path budget (``pass_paths``) is split over the active jobs
     by ``share_pass``.  2^24 paths keep the traversal kernels near their
     large-pass rate while a pass stays short enough (tens of ms) that new
-    arrivals join soon; the default policy, shortest remaining work first
-    with backfill, gave the lowest median latency at every offered load
-    measured, with a 99th percentile near the serial loop's.  Requests are
-    admitted in arrival order while job slots and the frame-memory budget
-    allow."""
+    arrivals join soon.  The default policy, shortest remaining work first
+    guarded by first-come-first-served deadlines, keeps the median latency
+    of shortest-first without letting the longest jobs finish later than
+    the arrival-order loop would.  Requests are admitted in arrival order
+    while job slots and the frame-memory budget allow."""
 
     def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=1 << 24, frame_bytes_budget=32 << 30,
-                 policy="srpt"):
+                 policy="deadline"):
         self.cache = cache
         self.policy = policy
         self.max_jobs = max_jobs
@@ -165,9 +243,8 @@ class RenderServer:
                 break
             self.waiting.popleft()
             pinned = [r.scene_key for r, _ in self.active]
-            scene, cam = self.cache.get(req.scene_key, pinned=pinned)
-            job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
-                            samples_per_pass=1, adaptive=req.adaptive, tag=req)
+            scene, cam, opts = self.cache.get(req.scene_key, pinned=pinned)
+            job = request_job(req, scene, cam, opts, samples_per_pass=1)
             req.admitted = now
             self.active.append((req, job))
             used += need
@@ -189,13 +266,14 @@ class RenderServer:
                              self.policy)
             for j, k in zip(jobs, spp):
                 j.samples_per_pass = k
-            batch = Batch(jobs, use_graph=False)
+            traced = [(r, j) for (r, j), k in zip(self.active, spp) if k > 0]
+            batch = Batch([j for _, j in traced], use_graph=False)
             batch.run(1)
             recs = batch.read_jobs()
             batch.close()
             self.passes += 1
-            still = []
-            for (req, job), rec in zip(self.active, recs):
+            still = [(r, j) for (r, j), k in zip(self.active, spp) if k == 0]
+            for (req, job), rec in zip(traced, recs):
                 job.first_sample = rec.first_sample
                 req.rays += int(rec.rays)
                 req.samples_done = int(rec.first_sample)
@@ -223,10 +301,9 @@ def serial_loop(cache: SceneCache, requests, paths_per_pass=TARGET_PATHS_PER_PAS
         while clock() - t0 < req.arrival:
             time.sleep(0.0005)
         req.admitted = clock() - t0
-        scene, cam = cache.get(req.scene_key)
+        scene, cam, opts = cache.get(req.scene_key)
         spp = int(max(1, min(req.samples, paths_per_pass // max(1, req.width * req.height))))
-        job = RenderJob(scene, cam, req.width, req.height, req.samples, max_depth=req.max_depth, seed=req.seed,
-                        samples_per_pass=spp, adaptive=req.adaptive, tag=req)
+        job = request_job(req, scene, cam, opts, samples_per_pass=spp)
         b = Batch([job], use_graph=False)
         b.run()
         rec = b.read_jobs()[0]

@@ -41,3 +41,85 @@ def test_served_results_equal_alone_results():
         assert r.rays == alone[r.rid].rays
     stats = latency_stats(done, max(r.finished for r in done))
     assert stats["completed"] == 6 and stats["p99_ms"] >= stats["p50_ms"] > 0
+
+
+# ---------------------------------------------------------------- serving workload
+# Requests drawn from the mesh generator (70k triangles) and a reduced
+# large-scene generator (40 displaced spheres + 16 analytic spheres + 3
+# lights, under the sky), 64^2..128^2, fixed sample counts or an error
+# target (the adaptive stop), arriving over time.
+def _c5_worlds():
+    from roxy_dcc_amd import scenes
+
+    return {("mesh", 20): scenes.mesh_scene(seed=20),
+            ("large", 30): scenes.large_scene(seed=30, n_meshes=40, seg_u=20, seg_v=11, n_spheres=16),
+            ("mesh", 21): scenes.mesh_scene(seed=21)}
+
+
+def _c5_requests():
+    from roxy_dcc_amd.serve import Request
+
+    spec = [(("mesh", 20), 64, 64, 6, None), (("large", 30), 128, 72, 4, None),
+            (("mesh", 21), 96, 96, 24, (0.02, 4, 2)), (("large", 30), 96, 64, 16, (0.004, 3, 1)),
+            (("mesh", 20), 128, 128, 3, None), (("mesh", 21), 80, 80, 5, None),
+            (("large", 30), 64, 64, 20, (0.0017, 5, 3)), (("mesh", 20), 72, 72, 8, None)]
+    return [Request(i, key, W, H, spp, arrival=0.004 * i, max_depth=6, seed=500 + i, adaptive=ad)
+            for i, (key, W, H, spp, ad) in enumerate(spec)]
+
+
+def test_serving_workload_matches_oracle_and_alone():
+    """Every served request equals the CPU oracle's render of that request
+    (same seed, the world's sky/background and NEE settings; for an error
+    target, the samples the adaptive rule of rendering/ray_tracer.py:531-541
+    takes on the oracle's own per-sample errors) pixel for pixel with the
+    same ray count, and equals its alone render."""
+    from oracle import oracle as orc
+    from roxy_dcc_amd.extract import extract_scene
+    from roxy_dcc_amd.serve import RenderServer, SceneCache, serial_loop
+    from tests import helpers
+
+    worlds = _c5_worlds()
+    flats = {}
+
+    def load(key):
+        world, _, _, st = worlds[key]
+        ex = extract_scene(world)
+        flats[key] = ex.flat
+        return ex.scene, world.active_camera, {"use_sky": bool(world._use_sky),
+                                               "background": tuple(world._background_color),
+                                               "direct_light_mode": st["direct_light_mode"],
+                                               "direct_light_max_depth": st["direct_light_max_depth"]}
+
+    # error targets chosen with a margin of 8% or more from every checked
+    # error of the oracle, which stops them after 16, 6 and 14 samples;
+    # a small pass budget: many passes, each request shares several of them
+    server = RenderServer(SceneCache(load, max_scenes=2), max_jobs=5, pass_paths=1 << 15)
+    done = server.run(_c5_requests())
+    assert len(done) == 8 and server.passes > 8
+    alone = {r.rid: r for r in serial_loop(SceneCache(load), _c5_requests())}
+    oscs = {}
+    for r in sorted(done, key=lambda q: q.rid):
+        world, _, _, st = worlds[r.scene_key]
+        if r.scene_key not in oscs:
+            oscs[r.scene_key] = helpers.flat_to_oracle(flats[r.scene_key])
+        osc = oscs[r.scene_key]
+        ocam = helpers.oracle_camera(world.active_camera, r.width, r.height)
+        oprm = orc.params(r.max_depth, int(world._use_sky), 0, st["direct_light_max_depth"], 10.0,
+                          world._background_color, r.seed)
+        n = r.samples
+        if r.adaptive is not None:
+            thr, amin, interval = r.adaptive
+            errs = orc.adaptive_errors(osc, ocam, oprm, r.samples, orc.Accum(r.width * r.height))
+            n = orc.adaptive_stop(errs, r.samples, amin, thr, interval)
+        acc = orc.Accum(r.width * r.height)
+        orc.render(osc, ocam, oprm, 0, n, acc)
+        got = r.frame.reshape(-1, 3)
+        same = float(np.all(got == acc.color, axis=1).mean())
+        print(f"request {r.rid} {r.scene_key} {r.width}x{r.height}: {r.samples_done} samples (oracle {n}), "
+              f"exact {same:.4f}, rays {r.rays} / {acc.counts.rays}, latency {r.latency * 1e3:.1f} ms", flush=True)
+        assert r.samples_done == n
+        np.testing.assert_array_equal(got, acc.color)
+        assert r.rays == acc.counts.rays
+        np.testing.assert_array_equal(r.frame, alone[r.rid].frame)
+        assert r.rays == alone[r.rid].rays
+    assert any(r.adaptive is not None and r.samples_done < r.samples for r in done), "no error target stopped early"

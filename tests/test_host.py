@@ -369,14 +369,14 @@ def test_share_pass_water_filling():
     from roxy_dcc_amd.serve import share_pass
 
     demands = [(64 * 64, 4), (512 * 512, 32), (1920 * 1080, 64), (128 * 128, 16)]
-    spp = share_pass(demands, 1 << 23)
+    spp = share_pass(demands, 1 << 23, "fair")
     assert all(1 <= s <= need for s, (_, need) in zip(spp, demands))
     assert spp[0] == 4 and spp[3] == 16
     assert sum(s * px for s, (px, _) in zip(spp, demands)) <= 1 << 23
     # one job alone takes as much of the budget as it can use
-    assert share_pass([(512 * 512, 64)], 1 << 23) == [32]
+    assert share_pass([(512 * 512, 64)], 1 << 23, "fair") == [32]
     # a frame larger than the budget still advances one sample per pass
-    assert share_pass([(1920 * 1080, 8)], 1 << 20) == [1]
+    assert share_pass([(1920 * 1080, 8)], 1 << 20, "fair") == [1]
     # first come first served with backfill: the oldest job takes what it
     # can use, the rest advance at least one sample
     fifo = share_pass(demands, 1 << 23, "fifo")
@@ -390,6 +390,42 @@ def test_share_pass_water_filling():
     assert share_pass(demands, 1 << 23, "srpt") == [4, 23, 1, 16]
     with pytest.raises(ValueError):
         share_pass(demands, 1 << 23, "lifo")
+
+
+def test_share_pass_deadline_guard():
+    """The default policy: shortest remaining work first unless a job would
+    then finish later than first come first served (plus the slack) lets it;
+    jobs whose next sample does not fit sit the pass out, the first in line
+    always advances, and the budget is never exceeded otherwise."""
+    from roxy_dcc_amd.serve import deadline_order, share_pass
+
+    demands = [(64 * 64, 4), (512 * 512, 32), (1920 * 1080, 64), (128 * 128, 16)]
+    spp = share_pass(demands, 1 << 23)
+    assert spp == [4, 30, 0, 16]
+    assert sum(s * px for s, (px, _) in zip(spp, demands)) <= 1 << 23
+    # a long job that arrived first: shortest-first while the slack allows,
+    # arrival order once the short job would push it past its deadline
+    work = [1920 * 1080 * 64, 512 * 512 * 32]
+    assert deadline_order(work, work[1]) == [1, 0]
+    assert deadline_order(work, work[1] - 1) == [0, 1]
+    assert deadline_order(work, 0) == [0, 1]
+    # a stream of small jobs behind a large one cannot overtake it for good
+    work = [1000] + [10] * 200
+    order = deadline_order(work, 100)
+    assert order.index(0) <= 10
+    # every order meets its deadlines
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        w = rng.integers(1, 1000, size=int(rng.integers(1, 12))).tolist()
+        slack = int(rng.integers(0, 500))
+        order = deadline_order(w, slack)
+        assert sorted(order) == list(range(len(w)))
+        t, dl = 0, np.cumsum(w) + slack
+        for k in order:
+            t += w[k]
+            assert t <= dl[k]
+    # a frame larger than the budget still advances one sample per pass
+    assert share_pass([(1920 * 1080, 8)], 1 << 20) == [1]
 
 
 def _random_indexed_mesh(rng, n_pos=200, n_tris=500, normals=True, uvs=True):
