@@ -425,7 +425,8 @@ def serving_bench(n_requests=40, loads=(0.5, 1.0, 2.0, 4.0), seed=5):
             server = RenderServer(cache, max_jobs=32, policy=policy)
             t0 = time.perf_counter()
             done = server.run(requests(arr))
-            row["server_" + policy] = dict(latency_stats(done, time.perf_counter() - t0), passes=server.passes)
+            row["server_" + policy] = dict(latency_stats(done, time.perf_counter() - t0), passes=server.passes,
+                                           pass_s=server.pass_s)
             same += sum(int(np.array_equal(r.frame, ref_frames[r.rid])) for r in done)
         row["identical_images"] = same
         rows.append(row)

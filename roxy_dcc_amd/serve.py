@@ -99,22 +99,25 @@ class SceneCache:
             self.evictions += 1
 
 
-def deadline_order(work, slack):
-    """Shortest remaining work first, guarded by first-come-first-served
-    deadlines.  ``work`` is each job's remaining paths in arrival order.
-    Serving the jobs one after another in arrival order would finish job k
-    after prefix(k) = work[0] + ... + work[k] paths: that, plus ``slack``
-    paths, is job k's deadline.  The order starts as shortest-first; while
-    some job would finish past its deadline, the first such job moves ahead
-    of the first job with a later deadline.  Arrival order meets every
-    deadline, so if the moves have not converged after n^2 of them, the
-    order falls back to it."""
-    n = len(work)
-    deadline = []
-    run = 0
+def fifo_deadlines(work, slack):
+    """Deadlines of jobs served one after another in arrival order: job k
+    finishes after work[0] + ... + work[k], plus ``slack``."""
+    out, run = [], 0
     for w in work:
         run += w
-        deadline.append(run + slack)
+        out.append(run + slack)
+    return out
+
+
+def deadline_order(work, deadline):
+    """Shortest remaining work first, guarded by deadlines.  ``work`` is
+    each job's remaining work and ``deadline`` the work done from now by
+    which it must finish, both in the same unit.  The order starts as
+    shortest-first; while some job would finish past its deadline, the
+    first such job moves ahead of the first job with a later deadline.  If
+    the moves have not converged after n^2 of them, the order is earliest
+    deadline first, which meets every deadline any order can meet."""
+    n = len(work)
     order = sorted(range(n), key=lambda k: (work[k], k))
     for _ in range(n * n + 1):
         t, late = 0, None
@@ -128,21 +131,25 @@ def deadline_order(work, slack):
         k = order.pop(late)
         ins = next((i for i, j in enumerate(order) if deadline[j] > deadline[k]), late)
         order.insert(ins, k)
-    return list(range(n))
+    return sorted(range(n), key=lambda k: (deadline[k], k))
 
 
-def share_pass(demands, budget, policy="deadline"):
+def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None):
     """Split of a pass's path budget: `demands` are (pixels, samples_left)
     per job in arrival order; returns samples per pass for each (whole
     samples, at most what the job still needs; 0 = not in this pass).
 
     "deadline" (the default): jobs take the budget in deadline_order
     (shortest remaining work first unless that would make some job finish
-    later than first-come-first-served would, with one pass of slack), each
-    as many samples as it can use; jobs whose next sample no longer fits
-    wait for a later pass, and the first job in the order always advances.
-    Low median latency like shortest-first, a tail no worse than the
-    arrival-order loop's.
+    past its deadline), each as many samples as it can use; jobs whose
+    next sample no longer fits wait for a later pass, and the first job in
+    the order always advances; a job whose remaining samples overshoot the
+    budget by at most a quarter of it takes them all.  A job's work is
+    pixels x samples left x its
+    cost per path (``costs``, 1 by default); ``deadlines`` are in the same
+    unit, counted from now (by default: first come first served from now,
+    with one pass of slack).  Low median latency like shortest-first, a
+    tail no worse than the arrival-order loop's.
     "fair": water-filling, smallest remaining work first, each job taking
     at most an equal share of what is left, so small jobs finish in one
     pass and the budget they leave goes to the larger ones (processor
@@ -156,13 +163,22 @@ def share_pass(demands, budget, policy="deadline"):
     n = len(demands)
     if policy == "deadline":
         out = [0] * n
-        order = deadline_order([px * need for px, need in demands], budget)
+        costs = costs or [1.0] * n
+        work = [px * need * c for (px, need), c in zip(demands, costs)]
+        stretch = 0.25
+        order = deadline_order(work, fifo_deadlines(work, budget) if deadlines is None else deadlines)
         left = budget
         for rank, k in enumerate(order):
             px, need = demands[k]
-            take = min(need, left // max(1, px))
+            take = max(0, min(need, left // max(1, px)))
             if rank == 0:
                 take = max(1, take)
+            # a job whose remainder overshoots the budget by a little finishes
+            # in this pass rather than taking a whole pass for its last
+            # samples (the pass grows by at most a quarter)
+            if take < need and (need - take) * px <= stretch * budget:
+                stretch -= (need - take) * px / budget
+                take = need
             out[k] = take
             left -= take * px
         return out
@@ -209,25 +225,62 @@ class RenderServer:
 
     Each iteration traces one wavefront pass of all active jobs together.
     The pass's path budget (``pass_paths``) is split over the active jobs
-    by ``share_pass``.  2^24 paths keep the traversal kernels near their
-    large-pass rate while a pass stays short enough (tens of ms) that new
-    arrivals join soon.  The default policy, shortest remaining work first
+    by ``share_pass``.  The drop-in renderer's pass size (2^25 paths) keeps
+    the traversal kernels at their large-pass rate, so the largest jobs are
+    traced as fast as the one-request-at-a-time loop traces them, while a
+    pass stays short enough (tens of ms) that new arrivals join soon.  The default policy, shortest remaining work first
     guarded by first-come-first-served deadlines, keeps the median latency
     of shortest-first without letting the longest jobs finish later than
     the arrival-order loop would.  Requests are admitted in arrival order
     while job slots and the frame-memory budget allow."""
 
-    def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=1 << 24, frame_bytes_budget=32 << 30,
-                 policy="deadline"):
+    def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=TARGET_PATHS_PER_PASS, frame_bytes_budget=32 << 30,
+                 policy="deadline", slack_passes=0.5):
         self.cache = cache
         self.policy = policy
         self.max_jobs = max_jobs
         self.pass_paths = pass_paths
+        # how far past its first-come-first-served finish a job may be put
+        # off for shorter ones, in passes (a pass cannot end exactly on a
+        # deadline)
+        self.slack_passes = slack_passes
         self.frame_bytes_budget = frame_bytes_budget
         self.waiting = collections.deque()
         self.active = []       # (Request, RenderJob)
         self.done = []
         self.passes = 0
+        # deadlines, in rays: the rays traced so far, the finish of the last
+        # arrival in a first-come-first-served schedule, rays per path by
+        # scene (measured on the passes traced so far)
+        self.rays_done = 0
+        self.fifo_finish = 0.0
+        self.rays_per_path = {}
+        self.deadline = {}
+        self._batch = None
+        self._batch_key = None
+        self._batch_rays = {}
+        self.pass_s = 0.0      # wall time inside run + read-back of the passes
+
+    def _close_batch(self):
+        if self._batch is not None:
+            self._batch.close()
+        self._batch = None
+        self._batch_key = None
+
+    def _cost(self, req):
+        return self.rays_per_path.get(req.scene_key, 0.5 * req.max_depth + 1.0)
+
+    def _note_arrivals(self, now):
+        """Each request, when it arrives, gets the finish time (in rays) that
+        serving every request one after another in arrival order would give
+        it: its deadline."""
+        for req in self.waiting:
+            if req.arrival > now:
+                break
+            if req.rid not in self.deadline:
+                work = req.width * req.height * req.samples * self._cost(req)
+                self.fifo_finish = max(self.fifo_finish, float(self.rays_done)) + work
+                self.deadline[req.rid] = self.fifo_finish
 
     @staticmethod
     def _frame_bytes(req):
@@ -257,34 +310,60 @@ class RenderServer:
         t0 = clock()
         while self.waiting or self.active:
             now = clock() - t0
+            self._note_arrivals(now)
             self._admit(now)
             if not self.active:
                 time.sleep(idle_sleep)
                 continue
             jobs = [j for _, j in self.active]
+            costs = [self._cost(r) for r, _ in self.active]
+            slack = self.slack_passes * self.pass_paths * sum(costs) / len(costs)
+            deadlines = [self.deadline[r.rid] - self.rays_done + slack for r, _ in self.active]
             spp = share_pass([(j.width * j.height, j.samples - j.first_sample) for j in jobs], self.pass_paths,
-                             self.policy)
+                             self.policy, deadlines=deadlines, costs=costs)
             for j, k in zip(jobs, spp):
                 j.samples_per_pass = k
             traced = [(r, j) for (r, j), k in zip(self.active, spp) if k > 0]
-            batch = Batch([j for _, j in traced], use_graph=False)
+            # the same jobs with the same shares as the last pass: trace the
+            # next pass of the same batch (its records advance on the device)
+            key = tuple((r.rid, j.samples_per_pass) for r, j in traced)
+            if self._batch is None or key != self._batch_key:
+                self._close_batch()
+                self._batch = Batch([j for _, j in traced], use_graph=False)
+                self._batch_key = key
+                self._batch_rays = {}
+            batch = self._batch
+            tr = clock()
             batch.run(1)
             recs = batch.read_jobs()
-            batch.close()
+            self.pass_s += clock() - tr
             self.passes += 1
             still = [(r, j) for (r, j), k in zip(self.active, spp) if k == 0]
+            finished = False
             for (req, job), rec in zip(traced, recs):
+                rays = int(rec.rays) - self._batch_rays.get(req.rid, 0)
+                self._batch_rays[req.rid] = int(rec.rays)
+                traced_paths = job.width * job.height * (int(rec.first_sample) - job.first_sample)
+                if traced_paths > 0 and rays > 0:
+                    rpp = rays / traced_paths
+                    old = self.rays_per_path.get(req.scene_key)
+                    self.rays_per_path[req.scene_key] = rpp if old is None else 0.5 * (old + rpp)
+                self.rays_done += rays
                 job.first_sample = rec.first_sample
-                req.rays += int(rec.rays)
+                req.rays += rays
                 req.samples_done = int(rec.first_sample)
                 if rec.samples_left <= 0:
+                    finished = True
                     torch.cuda.synchronize()
                     req.frame = job.frame.color.cpu().numpy()
                     req.finished = clock() - t0
                     self.done.append(req)
                 else:
                     still.append((req, job))
+            if finished:
+                self._close_batch()
             self.active = still
+        self._close_batch()
         return self.done
 
 

@@ -74,6 +74,11 @@ def test_reordered_queues_in_batches_change_nothing():
         b.run()
         torch.cuda.synchronize()
         imgs[name] = [_color(j) for j in jobs]
+        if graph:
+            # the captured pass holds the queue sorts (kernels only)
+            kernels, memsets, others = b.graph_nodes()
+            print(f"captured pass with sorts: {kernels} kernel nodes, {memsets} memset, {others} other", flush=True)
+            assert kernels > 40 and memsets == 0 and others == 0
         b.close()
     for g, d, k in zip(imgs["graph"], imgs["direct"], imgs["kept"]):
         print(f"mean {k.mean():.4f}: graph==kept {np.array_equal(g, k)} direct==kept {np.array_equal(d, k)}",

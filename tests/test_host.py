@@ -397,28 +397,37 @@ def test_share_pass_deadline_guard():
     then finish later than first come first served (plus the slack) lets it;
     jobs whose next sample does not fit sit the pass out, the first in line
     always advances, and the budget is never exceeded otherwise."""
-    from roxy_dcc_amd.serve import deadline_order, share_pass
+    from roxy_dcc_amd.serve import deadline_order, fifo_deadlines, share_pass
 
     demands = [(64 * 64, 4), (512 * 512, 32), (1920 * 1080, 64), (128 * 128, 16)]
     spp = share_pass(demands, 1 << 23)
-    assert spp == [4, 30, 0, 16]
-    assert sum(s * px for s, (px, _) in zip(spp, demands)) <= 1 << 23
+    # the 512^2 job's last two samples overshoot the budget by 3%: it
+    # finishes in this pass
+    assert spp == [4, 32, 0, 16]
+    assert sum(s * px for s, (px, _) in zip(spp, demands)) <= 1.25 * (1 << 23)
+    assert share_pass([(512 * 512, 41)], 1 << 23) == [32]      # over by more than 25%: not stretched
+    assert share_pass([(512 * 512, 36)], 1 << 23) == [36]
     # a long job that arrived first: shortest-first while the slack allows,
     # arrival order once the short job would push it past its deadline
     work = [1920 * 1080 * 64, 512 * 512 * 32]
-    assert deadline_order(work, work[1]) == [1, 0]
-    assert deadline_order(work, work[1] - 1) == [0, 1]
-    assert deadline_order(work, 0) == [0, 1]
+    assert deadline_order(work, fifo_deadlines(work, work[1])) == [1, 0]
+    assert deadline_order(work, fifo_deadlines(work, work[1] - 1)) == [0, 1]
+    assert deadline_order(work, fifo_deadlines(work, 0)) == [0, 1]
     # a stream of small jobs behind a large one cannot overtake it for good
     work = [1000] + [10] * 200
-    order = deadline_order(work, 100)
+    order = deadline_order(work, fifo_deadlines(work, 100))
     assert order.index(0) <= 10
+    # deadlines that no order meets: earliest deadline first
+    assert deadline_order([5, 5, 5], [4, 3, 20]) == [1, 0, 2]
+    # explicit deadlines and costs: the costly job is the long one
+    spp = share_pass([(100, 10), (100, 10)], 10 ** 6, deadlines=[10 ** 9, 10 ** 9], costs=[5.0, 1.0])
+    assert spp == [10, 10]
     # every order meets its deadlines
     rng = np.random.default_rng(3)
     for _ in range(200):
         w = rng.integers(1, 1000, size=int(rng.integers(1, 12))).tolist()
         slack = int(rng.integers(0, 500))
-        order = deadline_order(w, slack)
+        order = deadline_order(w, fifo_deadlines(w, slack))
         assert sorted(order) == list(range(len(w)))
         t, dl = 0, np.cumsum(w) + slack
         for k in order:
