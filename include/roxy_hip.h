@@ -376,6 +376,19 @@ int rx_render(const rx_scene* scene, const rx_camera* camera,
               const rx_accum* accum, rx_counters* d_counters,
               rx_profile* profile, void* stream);
 
+/* rx_render without a caller workspace, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace is a
+ * library-owned device array of RX_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; a pass needing more (rx_render_workspace_size)
+ * fails with RX_ERR_CAPACITY, so use fewer samples_per_pass.  No HIP
+ * allocation is made; RX_FLAG_PROFILE is rejected.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+#define RX_NOALLOC_SCRATCH_BYTES (768ull << 20)
+int rx_render_noalloc(const rx_scene* scene, const rx_camera* camera,
+                      const rx_render_params* params, int first_sample, int n_samples,
+                      const rx_accum* accum, rx_counters* d_counters, void* stream);
+
 /* ---- batches: many scenes in one wavefront ------------------------------
  * One job = one scene, camera, settings and accumulation target.  A batch
  * traces a pass of every unfinished job together: each pass renders

@@ -33,6 +33,7 @@ RX_FLAG_PRIM_PREPASS = 4
 RX_FLAG_SMALL_BLOCKS = 8
 RX_FLAG_ADAPTIVE = 16
 RX_FLAG_KEEP_QUEUE_ORDER = 32
+RX_NOALLOC_SCRATCH_BYTES = 768 << 20
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -126,6 +127,8 @@ SIGNATURES = {
     "rx_render_workspace_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "rx_render": (C.c_int, [C.POINTER(RxScene), C.POINTER(RxCamera), C.POINTER(RxRenderParams), C.c_int, C.c_int,
                             vp, C.c_size_t, C.POINTER(RxAccum), vp, C.POINTER(RxProfile), vp]),
+    "rx_render_noalloc": (C.c_int, [C.POINTER(RxScene), C.POINTER(RxCamera), C.POINTER(RxRenderParams), C.c_int,
+                                    C.c_int, C.POINTER(RxAccum), vp, vp]),
     "rx_batch_workspace_size": (C.c_int, [C.POINTER(RxJob), C.c_int, C.POINTER(C.c_size_t)]),
     "rx_batch_create": (C.c_int, [C.POINTER(RxJob), C.c_int, vp, C.c_size_t, C.c_int, C.POINTER(vp)]),
     "rx_batch_run": (C.c_int, [vp, C.c_int, vp, C.POINTER(RxProfile), vp]),

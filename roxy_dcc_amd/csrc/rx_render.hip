@@ -21,6 +21,7 @@
 // depend on the pass size, the queue order or the other jobs.
 #include <stdlib.h>
 
+#include <mutex>
 #include <new>
 
 #include "rx_launch.h"
@@ -2193,4 +2194,69 @@ extern "C" int rx_batch_destroy(rx_batch* b) {
     if (b->captured) (void)hipGraphExecDestroy(b->exec);
     delete b;
     return RX_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation render
+// rx_render with the workspace in a library-owned device array instead of
+// a caller buffer, for callers that cannot allocate (code under HIP graph
+// capture, serving loops, harnesses that forbid hipMalloc).  The array is
+// a file-scope device symbol: each GPU holds its own copy until the
+// process ends.  Its device address comes from hipGetSymbolAddress, once
+// per device on that device's first call (the array's name is only a host
+// stand-in), and only that address is handed to the kernels.  The kernels
+// clear every counter, queue length and partial sum they read before use
+// (k_set_job, k_clear_ray_part, k_pass_begin), so the array's previous
+// contents do not matter; calls that overlap on one GPU share it and race.
+static __device__ __align__(256) unsigned char scratch[RX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {nullptr};
+
+int scratch_address(void** out) {
+    int dev = 0;
+    RX_TRY_HIP(hipGetDevice(&dev));
+    RX_REQUIRE(dev >= 0 && dev < MAX_DEVICES, RX_ERR_INVALID, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        RX_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RX_REQUIRE(p, RX_ERR_HIP, "no device address for the no-allocation scratch array");
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return RX_OK;
+}
+
+}  // namespace
+
+extern "C" int rx_render_noalloc(const rx_scene* scene, const rx_camera* camera, const rx_render_params* params,
+                                 int first_sample, int n_samples, const rx_accum* accum, rx_counters* d_counters,
+                                 void* stream) {
+    RX_REQUIRE(scene && camera && params && accum && d_counters, RX_ERR_INVALID, "NULL argument");
+    RX_REQUIRE(!(params->flags & RX_FLAG_PROFILE), RX_ERR_INVALID,
+               "rx_render_noalloc does not time stages (RX_FLAG_PROFILE)");
+    // the workspace this pass size needs, from the same plan rx_render makes
+    rx_job j;
+    j.scene = *scene;
+    j.camera = *camera;
+    j.params = *params;
+    j.accum = *accum;
+    j.first_sample = first_sample;
+    j.samples_left = n_samples;
+    long long P = 0, NP = 0;
+    pass_plan pl;
+    int st = plan_jobs(&j, 1, &P, &NP, &pl, nullptr, nullptr);
+    if (st) return st;
+    const size_t need = workspace_bytes_for(1, P, pl.deep, pl.max_spp, true);
+    RX_REQUIRE(need <= RX_NOALLOC_SCRATCH_BYTES, RX_ERR_CAPACITY,
+               "a pass of %lld paths needs %zu bytes of workspace; rx_render_noalloc has %zu (use fewer "
+               "samples_per_pass)", P, need, (size_t)RX_NOALLOC_SCRATCH_BYTES);
+    void* ptr = nullptr;
+    st = scratch_address(&ptr);
+    if (st) return st;
+    return rx_render(scene, camera, params, first_sample, n_samples, ptr, RX_NOALLOC_SCRATCH_BYTES, accum,
+                     d_counters, nullptr, stream);
 }
