@@ -1778,15 +1778,16 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
             RX_STAGE(tm, 6, (k_ray_keys<<<gp, BS, 0, s>>>(w, qi, key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
             int rs = tm.begin(6);
             if (rs) return rs;
+            uint32_t* perm = nullptr;
             RX_TRY_HIP(enqueue_ray_sort(w.sk_in, w.si_in, w.sk_out, w.si_out, w.counts + qi, w.P, SORT_MIN_RAYS,
-                                        key_bits(w.n_jobs), w.sort_scratch, s));
+                                        key_bits(w.n_jobs), w.sort_scratch, s, nullptr, &perm));
             rs = tm.end();
             if (rs) return rs;
             // k_extend reads each ray through the permutation: cheaper
             // than gathering a sorted copy first (2.6 against 3.7 ms of
             // reordering per 64-spp mesh render, same traversal time)
-            we.ray_slot = w.si_in;
-            we.hit_slot = w.si_in;
+            we.ray_slot = perm;
+            we.hit_slot = perm;
         }
         if (pl.prim_prepass) {
             if (pl.count) {
@@ -1810,11 +1811,12 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
                 RX_STAGE(tm, 6, (k_nee_keys<<<gp, BS, 0, s>>>(w, d, nee_key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
                 int rs = tm.begin(6);
                 if (rs) return rs;
+                uint32_t* perm = nullptr;
                 RX_TRY_HIP(enqueue_ray_sort(w.sk_in, w.si_in, w.sk_out, w.si_out, w.counts + 2, w.P, SORT_MIN_RAYS,
-                                            nee_key_bits(w.n_jobs), w.sort_scratch, s));
+                                            nee_key_bits(w.n_jobs), w.sort_scratch, s, nullptr, &perm));
                 rs = tm.end();
                 if (rs) return rs;
-                wc.nee_slot = w.si_in;
+                wc.nee_slot = perm;
             }
             if (pl.count)
                 RX_STAGE(tm, 3, (k_connect<CBS, TLN, true><<<gc, CBS, 0, s>>>(wc, d)));
@@ -2050,8 +2052,13 @@ extern "C" int rx_radix_sort_pairs(uint32_t* d_keys, uint32_t* d_vals, uint32_t*
     RX_REQUIRE(key_bits >= 1 && key_bits <= 32, RX_ERR_INVALID, "key_bits must be in 1..32");
     RX_REQUIRE(scratch_bytes >= sort_scratch_words(cap) * sizeof(uint32_t), RX_ERR_WORKSPACE,
                "sort scratch too small");
+    uint32_t *rk = nullptr, *rv = nullptr;
     RX_TRY_HIP(enqueue_ray_sort(d_keys, d_vals, d_keys2, d_vals2, d_count, cap, min_n < 0 ? 0 : min_n,
-                                (unsigned)key_bits, (uint32_t*)d_scratch, rx_stream(stream)));
+                                (unsigned)key_bits, (uint32_t*)d_scratch, rx_stream(stream), &rk, &rv));
+    if (rk != d_keys) {
+        k_sort_copy<<<grid_for(cap, 256, 4), 256, 0, rx_stream(stream)>>>(rk, rv, d_keys, d_vals, d_count, cap);
+        RX_CHECK_LAUNCH();
+    }
     return RX_OK;
 }
 
