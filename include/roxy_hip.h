@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define RX_ABI_VERSION 3
+#define RX_ABI_VERSION 4
 
 /* status codes */
 #define RX_OK 0

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.lib()
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.rx_version() == 3
+    assert lib.rx_version() == 4
 
 
 def test_morton_shape_closed_form_matches_recursion():
