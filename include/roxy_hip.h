@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define RX_ABI_VERSION 4
+#define RX_ABI_VERSION 5
 
 /* status codes */
 #define RX_OK 0
@@ -370,6 +370,16 @@ int rx_any_hit(const rx_scene* scene, const float* d_rays, const float* d_tmax,
  * the call synchronises) only when RX_FLAG_PROFILE is set. */
 int rx_render_workspace_size(int width, int height, int samples_per_pass,
                              int bvh_depth, size_t* bytes);
+/* The same workspace with room past it for the eye-relative copies of the
+ * scene that rx_render's camera-ray traversal reads when they fit: node
+ * boxes and triangle vertices minus the camera position, with the terms of
+ * the intersection tests built from them alone (64 bytes per node, 48 per
+ * triangle, 32 per analytic primitive).  Same images and counters either
+ * way; a workspace of rx_render_workspace_size() bytes renders without
+ * them.  No counterpart in the reference (render_kernel recomputes these
+ * per ray, rendering/taichi/intersect.py:28-78). */
+int rx_render_workspace_size_for_scene(const rx_scene* scene, int width, int height,
+                                       int samples_per_pass, size_t* bytes);
 int rx_render(const rx_scene* scene, const rx_camera* camera,
               const rx_render_params* params, int first_sample, int n_samples,
               void* d_workspace, size_t workspace_bytes,

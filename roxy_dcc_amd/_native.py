@@ -125,6 +125,8 @@ SIGNATURES = {
     "rx_closest_hit": (C.c_int, [C.POINTER(RxScene), vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
     "rx_any_hit": (C.c_int, [C.POINTER(RxScene), vp, vp, C.c_int, vp, vp, vp]),
     "rx_render_workspace_size": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "rx_render_workspace_size_for_scene": (C.c_int, [C.POINTER(RxScene), C.c_int, C.c_int, C.c_int,
+                                                     C.POINTER(C.c_size_t)]),
     "rx_render": (C.c_int, [C.POINTER(RxScene), C.POINTER(RxCamera), C.POINTER(RxRenderParams), C.c_int, C.c_int,
                             vp, C.c_size_t, C.POINTER(RxAccum), vp, C.POINTER(RxProfile), vp]),
     "rx_render_noalloc": (C.c_int, [C.POINTER(RxScene), C.POINTER(RxCamera), C.POINTER(RxRenderParams), C.c_int,

@@ -319,6 +319,133 @@ __device__ __forceinline__ float cube_hit(f3 ro, f3 rd, f3 c, f3 e) {
     return t;
 }
 
+// ---------------------------------------------------------------- eye-relative tests
+// Camera rays all start at the eye e.  Their tests subtract e from every box
+// plane, triangle vertex and primitive centre, and some terms are built from
+// those differences alone; those are the same fp32 values for every camera
+// ray, so a pass computes them once (rx_render.hip: k_eye_prep) and the
+// camera-ray traversal reads them.  Each function below performs the rest of
+// its counterpart's operations above in the same order, so it returns
+// bit-identical results for a ray that starts at e.
+
+// slab() with each plane given as plane - e.
+__device__ __forceinline__ bool slab_rel(f3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
+                                         float limit, float* t_enter_out) {
+    float tx0 = lx * inv.x, tx1 = hx * inv.x;
+    float ty0 = ly * inv.y, ty1 = hy * inv.y;
+    float tz0 = lz * inv.z, tz1 = hz * inv.z;
+    float t_enter = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
+    float t_exit = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
+    *t_enter_out = t_enter;
+    return t_exit >= t_enter && t_exit >= 0.001f && t_enter <= limit;
+}
+
+// tri_hit() with s = e - v0 and q = cross(s, e1) given.
+__device__ __forceinline__ float tri_hit_rel(f3 rd, f3 s, f3 e1, f3 e2, f3 q, float* uo, float* vo) {
+    f3 h = cross(rd, e2);
+    float a = dot(e1, h);
+    float t = -1.0f, u = 0.0f, v = 0.0f;
+    if (fabsf(a) > 1e-8f) {
+        float f = 1.0f / a;
+        u = f * dot(s, h);
+        if (u >= 0.0f && u <= 1.0f) {
+            v = f * dot(rd, q);
+            if (v >= 0.0f && u + v <= 1.0f) {
+                float tc = f * dot(e2, q);
+                if (tc > 0.001f) t = tc;
+            }
+        }
+    }
+    *uo = u;
+    *vo = v;
+    return t;
+}
+
+// sphere_hit() with oc = e - c and cc = dot(oc, oc) - r * r given.
+__device__ __forceinline__ float sphere_hit_rel(f3 rd, f3 oc, float cc) {
+    float h = dot(rd, oc);
+    float disc = h * h - cc;
+    float t = -1.0f;
+    if (disc >= 0.0f) {
+        float sq = sqrtf(disc);
+        t = -h - sq;
+        if (t < 0.001f) t = -h + sq;
+        if (t < 0.001f) t = -1.0f;
+    }
+    return t;
+}
+
+// plane_hit() with k = dot(n, e) + off given.
+__device__ __forceinline__ float plane_hit_rel(f3 rd, f3 n, float k) {
+    float denom = dot(n, rd);
+    float t = -1.0f;
+    if (fabsf(denom) >= 1e-6f) {
+        float tc = -k / denom;
+        if (tc >= 0.001f) t = tc;
+    }
+    return t;
+}
+
+// cube_hit() with lo = (c - half) - e and hi = (c + half) - e given.
+__device__ __forceinline__ float cube_hit_rel(f3 rd, f3 lo, f3 hi) {
+    float tx0 = lo.x / rd.x, tx1 = hi.x / rd.x;
+    float ty0 = lo.y / rd.y, ty1 = hi.y / rd.y;
+    float tz0 = lo.z / rd.z, tz1 = hi.z / rd.z;
+    float t_enter = fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1)));
+    float t_exit = fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1)));
+    float t = -1.0f;
+    if (t_exit >= t_enter && t_exit >= 0.001f) t = t_enter >= 0.001f ? t_enter : t_exit;
+    return t;
+}
+
+// The eye-relative record of primitive p (32 bytes, written by k_eye_prep):
+// sphere {e - c, dot(e - c, e - c) - r * r}; plane {n, dot(n, e) + off};
+// cube {lo - e, hi.x - e.x}, {hi.y - e.y, hi.z - e.z}; the type's bits in
+// the last word.  Other types keep no eye-relative terms (EYE_PRIM_NONE)
+// and are tested from their own record.
+constexpr int EYE_PRIM_NONE = -1;
+__device__ __forceinline__ void eye_prim(const rx_prim& p, f3 e, gv4f& r0, gv4f& r1) {
+    f3 a = mk(p.a[0], p.a[1], p.a[2]);
+    r0 = gv4f{0.0f, 0.0f, 0.0f, 0.0f};
+    r1 = gv4f{0.0f, 0.0f, 0.0f, __int_as_float(p.type)};
+    switch (p.type) {
+        case RX_PRIM_SPHERE: {
+            f3 oc = e - a;
+            float r = p.a[3];
+            r0 = gv4f{oc.x, oc.y, oc.z, dot(oc, oc) - r * r};
+            break;
+        }
+        case RX_PRIM_PLANE:
+            r0 = gv4f{a.x, a.y, a.z, dot(a, e) + p.a[3]};
+            break;
+        case RX_PRIM_CUBE: {
+            f3 h = mk(p.b[0], p.b[1], p.b[2]);
+            f3 lo = a - h, hi = a + h;
+            r0 = gv4f{lo.x - e.x, lo.y - e.y, lo.z - e.z, hi.x - e.x};
+            r1 = gv4f{hi.y - e.y, hi.z - e.z, 0.0f, __int_as_float(p.type)};
+            break;
+        }
+        default:
+            r1.w = __int_as_float(EYE_PRIM_NONE);
+            break;
+    }
+}
+
+// prim_hit() of a camera ray from the eye-relative record (r0, r1) of a
+// type that has one (the caller tests the others from their own record).
+__device__ __forceinline__ float prim_hit_rel(gv4f r0, gv4f r1, f3 rd) {
+    switch (__float_as_int(r1.w)) {
+        case RX_PRIM_SPHERE:
+            return sphere_hit_rel(rd, mk(r0.x, r0.y, r0.z), r0.w);
+        case RX_PRIM_PLANE:
+            return plane_hit_rel(rd, mk(r0.x, r0.y, r0.z), r0.w);
+        case RX_PRIM_CUBE:
+            return cube_hit_rel(rd, mk(r0.x, r0.y, r0.z), mk(r0.w, r1.x, r1.y));
+        default:
+            return -1.0f;
+    }
+}
+
 // scene/primitives.py:203-215
 __device__ __forceinline__ f3 cube_normal(f3 p, f3 c, f3 e) {
     f3 d = mk((p.x - c.x) / e.x, (p.y - c.y) / e.y, (p.z - c.z) / e.z);
@@ -418,11 +545,8 @@ struct lane_stack {
 // stack round trip fewer per node.  Returns false when neither child box
 // is hit (the caller pops).
 template <class Stack>
-__device__ __forceinline__ bool visit_boxes(gv4f a, gv4f b, gv4f c, gv2i d, f3 ro, f3 inv, float limit, Stack& stk,
-                                            int& sp, int& next) {
-    float lt, rt;
-    bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
-    bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
+__device__ __forceinline__ bool take_children(bool lh, bool rh, float lt, float rt, gv2i d, Stack& stk, int& sp,
+                                              int& next) {
     if (lh && rh) {
         if (lt < rt) {
             stk.put(sp++, d.y, rt);
@@ -445,6 +569,33 @@ __device__ __forceinline__ bool visit_boxes(gv4f a, gv4f b, gv4f c, gv2i d, f3 r
 }
 
 template <class Stack>
+__device__ __forceinline__ bool visit_boxes(gv4f a, gv4f b, gv4f c, gv2i d, f3 ro, f3 inv, float limit, Stack& stk,
+                                            int& sp, int& next) {
+    float lt, rt;
+    bool lh = slab(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
+    bool rh = slab(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
+    return take_children(lh, rh, lt, rt, d, stk, sp, next);
+}
+
+// visit_boxes() of a camera ray on a node whose planes are given minus the
+// eye (the camera-ray traversal's eye-relative node copy).
+template <class Stack>
+__device__ __forceinline__ bool visit_boxes_rel(gv4f a, gv4f b, gv4f c, gv2i d, f3 inv, float limit, Stack& stk,
+                                                int& sp, int& next) {
+    float lt, rt;
+    bool lh = slab_rel(inv, a.x, a.y, a.z, a.w, b.x, b.y, limit, &lt);
+    bool rh = slab_rel(inv, b.z, b.w, c.x, c.y, c.z, c.w, limit, &rt);
+    return take_children(lh, rh, lt, rt, d, stk, sp, next);
+}
+
+// The planes of the three 16-byte words of a node's child boxes (minima
+// x y z, maxima x y z, twice): the eye's coordinates in that order, so
+// word - eye_planes(k % 3, e) is word k minus the eye.
+__device__ __forceinline__ gv4f eye_planes(int part, f3 e) {
+    return part == 0 ? gv4f{e.x, e.y, e.z, e.x} : (part == 1 ? gv4f{e.y, e.z, e.x, e.y} : gv4f{e.z, e.x, e.y, e.z});
+}
+
+template <class Stack>
 __device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes, int link, f3 ro, f3 inv,
                                                float limit, Stack& stk, int& sp, int& next) {
     // the two child boxes and the two links: 56 of the record's 64 bytes
@@ -458,7 +609,10 @@ __device__ __forceinline__ bool visit_interior(const rx_node* __restrict__ nodes
 // links, already rewritten to slots where the child has one, at link[k].
 // The same boxes, links and tests as the node record, so the visit
 // sequence is visit_interior's.
-constexpr int TL_NODES = 511;
+#ifndef RX_TL_NODES
+#define RX_TL_NODES 511
+#endif
+constexpr int TL_NODES = RX_TL_NODES;
 constexpr int TL_REF = 1 << 30;
 
 template <class Stack>
@@ -489,6 +643,16 @@ __device__ __forceinline__ void load_tri(const rx_tri* __restrict__ tris, int i,
     v0 = mk(p0.x, p0.y, p0.z);
     e1 = mk(p0.w, p1.x, p1.y);
     e2 = mk(p1.z, p1.w, p2);
+}
+
+// An eye-relative triangle record {s = e - v0, e1, e2, q = cross(s, e1)}.
+__device__ __forceinline__ void load_tri_rel(const rx_tri* __restrict__ tris, int i, f3& s, f3& e1, f3& e2, f3& q) {
+    const RX_GLOBAL gv4f* tp = (const RX_GLOBAL gv4f*)(tris + i);
+    gv4f p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    s = mk(p0.x, p0.y, p0.z);
+    e1 = mk(p0.w, p1.x, p1.y);
+    e2 = mk(p1.z, p1.w, p2.x);
+    q = mk(p2.y, p2.z, p2.w);
 }
 
 // Closest hit over the BVH: rendering/taichi/intersect.py:81-152.  Visit

@@ -73,7 +73,10 @@ inline int trav_grid(long long n, bool deep, int per_cu = 0) {
 // 1024 x 6 x 8 B + 511 x 56 B = 76.6 KB and 512 x 6 x 8 B + 28.6 KB =
 // 52.6 KB, within a CU's 160 KB at those block counts (TL_NODES and
 // TL_REF are in rx_device.h).
-constexpr int STACK_LDS_TL = 6;
+#ifndef RX_STACK_LDS_TL
+#define RX_STACK_LDS_TL 6
+#endif
+constexpr int STACK_LDS_TL = RX_STACK_LDS_TL;
 constexpr int BS_EXTEND = 1024;
 constexpr int EXTEND_BLOCKS_PER_CU = 2;
 constexpr int BS_CONNECT = 512;

@@ -118,6 +118,15 @@ struct ws_view {
     gv2i* tl_link;
     int32_t* tl_count;
     const rx_node* tl_nodes;
+    // eye-relative copies of the (single) job's scene for its camera rays
+    // (k_eye_prep; null when the workspace has no room for them or the pass
+    // holds several jobs): node boxes minus the eye, triangles as {e - v0,
+    // e1, e2, cross(e - v0, e1)}, the root record's box minus the eye, and a
+    // 32-byte record per primitive (eye_prim)
+    rx_node* eye_nodes;
+    rx_tri* eye_tris;
+    rx_bvh_header* eye_header;
+    gv4f* eye_prims;
 };
 
 void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
@@ -127,6 +136,10 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.tl_link = ar.take<gv2i>(TL_NODES);
     w.tl_count = ar.take<int32_t>(4);
     w.tl_nodes = nullptr;
+    w.eye_nodes = nullptr;
+    w.eye_tris = nullptr;
+    w.eye_header = nullptr;
+    w.eye_prims = nullptr;
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
@@ -648,6 +661,82 @@ __global__ __launch_bounds__(BS) void k_prim_scan(ws_view w, int qi) {
     if (COUNT) wave_add(&ctr_slot(w)->prim_tests, cnt.prims);
 }
 
+// ---------------------------------------------------------------- eye-relative copies
+// The camera rays of a single-job render all start at the camera position
+// e.  k_eye_prep writes, once per call, what their tests compute first from
+// e alone: node boxes minus e, triangles as {e - v0, e1, e2, cross(e - v0,
+// e1)}, the root box minus e and the primitives' eye-relative records
+// (eye_prim); the camera-ray launch of k_extend reads these instead
+// (slab_rel, tri_hit_rel, prim_hit_rel: the remaining operations of the
+// same tests, so the same bits).
+__global__ __launch_bounds__(BS) void k_eye_prep(ws_view w) {
+    const rx_job& J = w.jobs[0];
+    const rx_scene& sc = J.scene;
+    const f3 e = ld3(J.camera.pos);
+    const int nn = sc.n_tris > 0 ? sc.n_nodes : 0;
+    long long n = nn > sc.n_tris ? nn : sc.n_tris;
+    if (sc.n_prims > n) n = sc.n_prims;
+    if (n < 1) n = 1;
+    for (long long k = (long long)blockIdx.x * BS + threadIdx.x; k < n; k += (long long)gridDim.x * BS) {
+        if (k < nn) {
+            const RX_GLOBAL gv4f* src = (const RX_GLOBAL gv4f*)(sc.nodes + k);
+            RX_GLOBAL gv4f* dst = (RX_GLOBAL gv4f*)(w.eye_nodes + k);
+            dst[0] = src[0] - eye_planes(0, e);
+            dst[1] = src[1] - eye_planes(1, e);
+            dst[2] = src[2] - eye_planes(2, e);
+            dst[3] = src[3];  // the links
+        }
+        if (k < sc.n_tris) {
+            f3 v0, e1, e2;
+            load_tri(sc.tris, (int)k, v0, e1, e2);
+            const f3 sv = e - v0;         // tri_hit: s = ro - v0
+            const f3 q = cross(sv, e1);   // tri_hit: q = cross(s, e1)
+            RX_GLOBAL gv4f* dst = (RX_GLOBAL gv4f*)(w.eye_tris + k);
+            dst[0] = gv4f{sv.x, sv.y, sv.z, e1.x};
+            dst[1] = gv4f{e1.y, e1.z, e2.x, e2.y};
+            dst[2] = gv4f{e2.z, q.x, q.y, q.z};
+        }
+        if (k < sc.n_prims) {
+            gv4f r0, r1;
+            eye_prim(gload(sc.prims + k), e, r0, r1);
+            ((RX_GLOBAL gv4f*)w.eye_prims)[2 * k] = r0;
+            ((RX_GLOBAL gv4f*)w.eye_prims)[2 * k + 1] = r1;
+        }
+        if (k == 0 && sc.n_tris > 0) {
+            rx_bvh_header h = gload(sc.header);
+            const float ec[3] = {e.x, e.y, e.z};
+            for (int a = 0; a < 3; ++a) {
+                h.bmin[a] = h.bmin[a] - ec[a];
+                h.bmax[a] = h.bmax[a] - ec[a];
+            }
+            *w.eye_header = h;
+        }
+    }
+}
+
+// closest_prim() of a camera ray of job 0 from the eye-relative records
+// (scalar loads: every lane reads the same record).
+template <bool COUNT>
+__device__ __forceinline__ void closest_prim_eye(const ws_view& w, f3 ro, f3 rd, float& prim_t, int& prim_id,
+                                                 trav_count& cnt) {
+    const RX_CONST rx_job* J = (const RX_CONST rx_job*)(w.jobs);
+    const int np = J->scene.n_prims;
+    const rx_prim* prims = J->scene.prims;
+    const RX_CONST gv4f* ep = (const RX_CONST gv4f*)w.eye_prims;
+    prim_t = 1e9f;
+    prim_id = -1;
+    for (int k = 0; k < np; ++k) {
+        const gv4f r0 = ep[2 * k], r1 = ep[2 * k + 1];
+        const float t = __float_as_int(r1.w) == EYE_PRIM_NONE ? prim_hit(cload(prims + k), ro, rd)
+                                                              : prim_hit_rel(r0, r1, rd);
+        if (COUNT) cnt.prims++;
+        if (t > 0.001f && t < prim_t) {
+            prim_t = t;
+            prim_id = k;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- treelet
 // The top of the pass's BVH for the traversal kernels' LDS: the first
 // TL_NODES interior nodes in breadth-first order, root in slot 0, each as
@@ -719,12 +808,19 @@ __global__ __launch_bounds__(TLB) void k_treelet(const rx_node* __restrict__ nod
 
 // A block's LDS copy of the treelet (TLN = 0: none).  Returns the slot
 // count, 0 when the pass has no treelet.
-template <int SBS, int TLN>
+// EYE: the boxes minus the eye of job 0 (the camera-ray traversal's
+// eye-relative form, as the node copy k_eye_prep writes).
+template <int SBS, int TLN, bool EYE = false>
 __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* link) {
     static_assert(TLN == 0 || TLN == TL_NODES, "the treelet's links address TL_NODES slots");
     if (TLN == 0 || w.tl_nodes == nullptr) return 0;
     const int n = w.tl_count[0];
-    for (int k = threadIdx.x; k < 3 * n; k += SBS) box[k] = ((const RX_GLOBAL gv4f*)w.tl_box)[k];
+    const f3 e = EYE ? ld3(w.jobs[0].camera.pos) : mk(0.0f, 0.0f, 0.0f);
+    for (int k = threadIdx.x; k < 3 * n; k += SBS) {
+        gv4f v = ((const RX_GLOBAL gv4f*)w.tl_box)[k];
+        if (EYE) v = v - eye_planes(k % 3, e);
+        box[k] = v;
+    }
     for (int k = threadIdx.x; k < n; k += SBS) link[k] = ((const RX_GLOBAL gv2i*)w.tl_link)[k];
     __syncthreads();
     return n;
@@ -742,14 +838,16 @@ __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* l
 // ray's own sequence of pops, tests and pushes is the one of
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
-template <int SBS, int TLN, bool COUNT, bool PRE>
+// EYE: the camera-ray launch of a single-job render, on the eye-relative
+// copies (k_eye_prep).
+template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi) {
     constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
     __shared__ int2 stack_mem[SL * SBS];
     __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
     __shared__ gv2i tl_link_mem[TLN > 0 ? TLN : 1];
     lane_stack<SBS, SL> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x, (int)gridDim.x * SBS};
-    const int tl_n = load_treelet<SBS, TLN>(w, tl_box_mem, tl_link_mem);
+    const int tl_n = load_treelet<SBS, TLN, EYE>(w, tl_box_mem, tl_link_mem);
     const RX_LDS gv4f* tl_box = (const RX_LDS gv4f*)tl_box_mem;
     const RX_LDS gv2i* tl_link = (const RX_LDS gv2i*)tl_link_mem;
     const int n = w.counts[qi];
@@ -790,6 +888,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[w.hit_slot ? (int)w.hit_slot[i] : i];
                     prim_t = h.x;
                     prim_id = __float_as_int(h.y);
+                } else if (EYE) {
+                    closest_prim_eye<COUNT>(w, ro, rd, prim_t, prim_id, cnt);
                 } else {
                     closest_prim_lanes<COUNT>(w, jx, ro, rd, prim_t, prim_id, cnt);
                 }
@@ -799,15 +899,18 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 best_v = 0.0f;
                 have = false;
                 if (sc->n_tris > 0) {
-                    nodes = sc->nodes;
-                    tris = sc->tris;
-                    const rx_bvh_header hdr = gload(sc->header);
+                    nodes = EYE ? w.eye_nodes : sc->nodes;
+                    tris = EYE ? w.eye_tris : sc->tris;
+                    const rx_bvh_header hdr = gload(EYE ? w.eye_header : sc->header);
                     inv = inv_dir(rd);
                     float rt;
-                    if (slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0], hdr.bmax[1], hdr.bmax[2],
-                             best_t, &rt)) {
+                    const bool root = EYE ? slab_rel(inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0],
+                                                     hdr.bmax[1], hdr.bmax[2], best_t, &rt)
+                                          : slab(ro, inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0],
+                                                 hdr.bmax[1], hdr.bmax[2], best_t, &rt);
+                    if (root) {
                         have = true;
-                        cur = TLN > 0 && tl_n > 0 && nodes == w.tl_nodes ? TL_REF : hdr.root_link;
+                        cur = TLN > 0 && tl_n > 0 && sc->nodes == w.tl_nodes ? TL_REF : hdr.root_link;
                         sp = 0;
                     }
                 }
@@ -838,7 +941,9 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                             c = q[2];
                             dl = ((const RX_GLOBAL gv2i*)q)[6];
                         }
-                        if (!visit_boxes(a, b, c, dl, ro, inv, best_t, stk, sp, cur)) have = pop_next(stk, sp, best_t, cur);
+                        const bool kids = EYE ? visit_boxes_rel(a, b, c, dl, inv, best_t, stk, sp, cur)
+                                              : visit_boxes(a, b, c, dl, ro, inv, best_t, stk, sp, cur);
+                        if (!kids) have = pop_next(stk, sp, best_t, cur);
                     }
                 }
                 if (have && cur < 0) {
@@ -846,10 +951,16 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     int lw = ~cur;
                     int start = lw >> 8, end = start + (lw & 255);
                     for (int t2 = start; t2 < end; ++t2) {
-                        f3 v0, e1, e2;
-                        load_tri(tris, t2, v0, e1, e2);
-                        float u, v;
-                        float t = tri_hit(ro, rd, v0, e1, e2, &u, &v);
+                        float u, v, t;
+                        if (EYE) {
+                            f3 sv, e1, e2, q;
+                            load_tri_rel(tris, t2, sv, e1, e2, q);
+                            t = tri_hit_rel(rd, sv, e1, e2, q, &u, &v);
+                        } else {
+                            f3 v0, e1, e2;
+                            load_tri(tris, t2, v0, e1, e2);
+                            t = tri_hit(ro, rd, v0, e1, e2, &u, &v);
+                        }
                         if (COUNT) cnt.tris++;
                         if (0.001f < t && t < best_t) {
                             best_t = t;
@@ -1789,19 +1900,26 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
             we.ray_slot = perm;
             we.hit_slot = perm;
         }
+        // the camera rays of a single-job pass start at its eye: its
+        // eye-relative copies serve them when the workspace holds them
+        const bool eye = d == 0 && w.eye_header != nullptr;
+#define RX_EXTEND(C, PR)                                                              \
+    (eye ? (k_extend<EBS, TLN, C, PR, true><<<ge, EBS, 0, s>>>(we, qi))               \
+         : (k_extend<EBS, TLN, C, PR, false><<<ge, EBS, 0, s>>>(we, qi)))
         if (pl.prim_prepass) {
             if (pl.count) {
                 RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<EBS, TLN, true, true><<<ge, EBS, 0, s>>>(we, qi)));
+                RX_STAGE(tm, 1, RX_EXTEND(true, true));
             } else {
                 RX_STAGE(tm, 5, (k_prim_scan<false><<<gp, BS, 0, s>>>(w, qi)));
-                RX_STAGE(tm, 1, (k_extend<EBS, TLN, false, true><<<ge, EBS, 0, s>>>(we, qi)));
+                RX_STAGE(tm, 1, RX_EXTEND(false, true));
             }
         } else if (pl.count) {
-            RX_STAGE(tm, 1, (k_extend<EBS, TLN, true, false><<<ge, EBS, 0, s>>>(we, qi)));
+            RX_STAGE(tm, 1, RX_EXTEND(true, false));
         } else {
-            RX_STAGE(tm, 1, (k_extend<EBS, TLN, false, false><<<ge, EBS, 0, s>>>(we, qi)));
+            RX_STAGE(tm, 1, RX_EXTEND(false, false));
         }
+#undef RX_EXTEND
         RX_STAGE(tm, 2, (k_shade<<<grid_for(w.P, SHADE_BS, 16 * BS / SHADE_BS), SHADE_BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             ws_view wc = w;
@@ -1945,6 +2063,40 @@ int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long lo
     return RX_OK;
 }
 
+// The eye-relative copies of a scene (k_eye_prep), after a render's own
+// workspace: the root record, the nodes, the triangles, 32 bytes per
+// primitive.
+void take_eye(rx_arena& ar, const rx_scene& sc, ws_view& w) {
+    w.eye_header = ar.take<rx_bvh_header>(1);
+    w.eye_nodes = ar.take<rx_node>(sc.n_tris > 0 ? (size_t)sc.n_nodes : 0);
+    w.eye_tris = ar.take<rx_tri>((size_t)sc.n_tris);
+    w.eye_prims = ar.take<gv4f>(2 * (size_t)sc.n_prims);
+}
+
+size_t eye_bytes_for(const rx_scene& sc) {
+    rx_arena ar{nullptr, 0, 0};
+    ws_view w;
+    take_eye(ar, sc, w);
+    return ar.off + 256;
+}
+
+// Camera rays the call traces against the records the copies take: they
+// pay for themselves when each record is read by many camera rays (and
+// the copy is a single streaming pass).
+bool eye_worthwhile(const rx_scene& sc, long long camera_rays) {
+    const long long records = (sc.n_tris > 0 ? (long long)sc.n_nodes + sc.n_tris : 0) + sc.n_prims;
+    return records > 0 && camera_rays >= 4 * records;
+}
+
+// Carves the copies from the workspace past its first `used` bytes; false
+// (and no copies) when they do not fit.
+bool carve_eye(void* d_workspace, size_t bytes, size_t used, const rx_scene& sc, ws_view& w) {
+    if (bytes < used || bytes - used < eye_bytes_for(sc)) return false;
+    rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, used};
+    take_eye(ar, sc, w);
+    return true;
+}
+
 }  // namespace
 
 struct rx_batch {
@@ -1992,6 +2144,18 @@ extern "C" int rx_render_workspace_size(int width, int height, int samples_per_p
     return RX_OK;
 }
 
+extern "C" int rx_render_workspace_size_for_scene(const rx_scene* scene, int width, int height, int samples_per_pass,
+                                                  size_t* bytes) {
+    RX_REQUIRE(scene && bytes, RX_ERR_INVALID, "NULL argument");
+    RX_REQUIRE(scene->n_nodes >= 0 && scene->n_tris >= 0 && scene->n_prims >= 0, RX_ERR_INVALID,
+               "negative scene size");
+    const int depth = scene->n_tris > 0 ? scene->bvh_depth : 0;
+    const int st = rx_render_workspace_size(width, height, samples_per_pass, depth, bytes);
+    if (st) return st;
+    *bytes += eye_bytes_for(*scene);
+    return RX_OK;
+}
+
 extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const rx_render_params* params,
                          int first_sample, int n_samples, void* d_workspace, size_t workspace_bytes,
                          const rx_accum* accum, rx_counters* d_counters, rx_profile* profile, void* stream) {
@@ -2017,11 +2181,24 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     ws_view w;
     st = carve_into(d_workspace, workspace_bytes, 1, P, NP, pl.deep, pl.max_spp, w, true);
     if (st) return st;
+    // eye-relative copies for the camera rays, when the workspace has room
+    // past the render's own part (rx_render_workspace_size_for_scene)
+    const long long camera_rays = (long long)camera->width * camera->height * n_samples;
+    const bool eye = eye_worthwhile(j.scene, camera_rays) &&
+                     carve_eye(d_workspace, workspace_bytes, workspace_bytes_for(1, P, pl.deep, pl.max_spp, true),
+                               j.scene, w);
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
     RX_CHECK_LAUNCH();
     k_clear_ray_part<<<1, 64, 0, s>>>(w);
     RX_CHECK_LAUNCH();
+    if (eye) {
+        long long recs = j.scene.n_tris > 0 ? (long long)j.scene.n_nodes : 0;
+        if (j.scene.n_tris > recs) recs = j.scene.n_tris;
+        if (j.scene.n_prims > recs) recs = j.scene.n_prims;
+        k_eye_prep<<<grid_for(recs > 0 ? recs : 1, BS, 16), BS, 0, s>>>(w);
+        RX_CHECK_LAUNCH();
+    }
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;
     tm.s = s;

@@ -61,17 +61,27 @@ class DeviceFrame:
 
 
 class Integrator:
-    """Wavefront path tracer over one DeviceScene (``rx_render``)."""
+    """Wavefront path tracer over one DeviceScene (``rx_render``).
 
-    def __init__(self, scene, width, height, samples_per_pass, device="cuda"):
+    ``eye_copies``: size the workspace for the scene's eye-relative copies
+    (``rx_render_workspace_size_for_scene``), which the camera-ray traversal
+    reads instead of recomputing box - eye, eye - v0 and the primitive
+    terms per ray; False renders the same images without them."""
+
+    def __init__(self, scene, width, height, samples_per_pass, device="cuda", eye_copies=True):
         import torch
 
         self.scene = scene
         self.width, self.height = int(width), int(height)
         self.samples_per_pass = int(samples_per_pass)
         ws = C.c_size_t()
-        nat.check(nat.lib().rx_render_workspace_size(self.width, self.height, self.samples_per_pass,
-                                                     scene.builder.depth, C.byref(ws)), "render workspace")
+        if eye_copies:
+            nat.check(nat.lib().rx_render_workspace_size_for_scene(C.byref(scene.struct()), self.width, self.height,
+                                                                   self.samples_per_pass, C.byref(ws)),
+                      "render workspace")
+        else:
+            nat.check(nat.lib().rx_render_workspace_size(self.width, self.height, self.samples_per_pass,
+                                                         scene.builder.depth, C.byref(ws)), "render workspace")
         self.workspace = torch.empty(ws.value, dtype=torch.uint8, device=device)
         self.counters = torch.zeros(8, dtype=torch.int64, device=device)
         self.profile = nat.RxProfile()
