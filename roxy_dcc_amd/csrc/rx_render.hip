@@ -70,6 +70,11 @@ struct ws_view {
     float *thr_x, *thr_y, *thr_z, *col_x, *col_y, *col_z, *ior;
     float *fn_x, *fn_y, *fn_z, *fa_x, *fa_y, *fa_z, *fd;
     int32_t *p_job, *p_rays;
+    // rays are counted per path (p_rays, summed by k_accumulate over the
+    // samples a job keeps) when a pass holds several jobs or an adaptive
+    // job, whose stop drops samples; otherwise the traversal kernels add
+    // their wave totals to job 0's ray partials directly (set per pass)
+    int ray_per_path;
     // two ray queues: 32 B per slot, {origin, path index bits} {direction, 0}
     gv4f* q_ray[2];
     // hits of the current queue: 16 B per slot, {t, coded id bits, u, v}
@@ -155,6 +160,7 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     for (float** f : pf) *f = ar.take<float>(P);
     w.p_job = ar.take<int32_t>(P);
     w.p_rays = ar.take<int32_t>(P);
+    w.ray_per_path = 1;
     for (int b = 0; b < 2; ++b) w.q_ray[b] = ar.take<gv4f>(2 * P);
     w.h_rec = ar.take<gv4f>(P);
     w.e_rec = ar.take<gv4f>(4 * P);
@@ -518,7 +524,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 w.fa_z[p] = 0.0f;
                 w.fd[p] = 0.0f;
                 w.p_job[p] = jx;
-                w.p_rays[p] = 0;
+                if (w.ray_per_path) w.p_rays[p] = 0;
             }
         }
         if (active) {
@@ -985,13 +991,14 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     v = best_v;
                 }
                 put_hit(w, w.hit_slot ? (int)w.hit_slot[i] : i, t, kind, id, u, v);
-                w.p_rays[p] += 1;
+                if (w.ray_per_path) w.p_rays[p] += 1;
                 i = -1;
             }
         }
     }
     rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, rays);
+    if (!w.ray_per_path) wave_add(w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs, rays);
     if (COUNT) {
         wave_add(&ctr->node_pops, cnt.pops);
         wave_add(&ctr->tri_tests, cnt.tris);
@@ -1423,7 +1430,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                 w.col_x[p] = w.col_x[p] + add.x;
                 w.col_y[p] = w.col_y[p] + add.y;
                 w.col_z[p] = w.col_z[p] + add.z;
-                w.p_rays[p] += cast;
+                if (w.ray_per_path) w.p_rays[p] += cast;
                 shadow += cast;
                 i = -1;
             }
@@ -1432,6 +1439,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, shadow);
     wave_add(&ctr->shadow_rays, shadow);
+    if (!w.ray_per_path) wave_add(w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs, shadow);
     if (COUNT) {
         wave_add(&ctr->shadow_node_pops, cnt.pops);
         wave_add(&ctr->shadow_tri_tests, cnt.tris);
@@ -1587,7 +1595,7 @@ __global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
                         aa = (aa * f + as_) / f1;
                         dd = (dd * f + ds) / f1;
                     }
-                    rays += (unsigned long long)w.p_rays[p];
+                    if (w.ray_per_path) rays += (unsigned long long)w.p_rays[p];
                 }
                 acc.color[3 * pix] = c.x;
                 acc.color[3 * pix + 1] = c.y;
@@ -1870,6 +1878,7 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     int gc = DEEP ? ge : render_trav_grid(w0.P, BS_CONNECT, CONNECT_BLOCKS_PER_CU, pl.trav_per_cu);
     ws_view w = w0;
     w.tl_nodes = TLN > 0 ? pl.tl_nodes : nullptr;
+    w.ray_per_path = w.n_jobs > 1 || pl.adaptive ? 1 : 0;
     int gp = grid_for(w.P, BS, 16);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     if (w.tl_nodes) RX_STAGE(tm, 0, (k_treelet<<<1, TLB, 0, s>>>(pl.tl_nodes, pl.tl_header, w)));
