@@ -8,7 +8,9 @@ with both methods:
 
   * "morton": _morton_order / _build_morton_node (bvh_builder.py:412-486)
   * "median": _build_median_node (bvh_builder.py:488-525), whose leaf order
-    is numpy's argpartition (introselect) at :510
+    is numpy's argpartition at :510 — which depends on the CPU numpy runs
+    on (argpartition_fingerprint), so the file records the fingerprint of
+    the machine that wrote it
 
 Cases: random triangles of many sizes (leaf-size boundaries 1, 8, 9, and
 non-powers of two), centroids on a coarse grid (equal Morton codes, equal
@@ -118,6 +120,21 @@ def tree_arrays(builder):
     }
 
 
+def argpartition_fingerprint():
+    """SHA-256 of np.argpartition's permutations of fixed float32 probes.
+    numpy picks its partition routine by the CPU's SIMD features (an
+    AVX-512 argselect, AVX2 or the scalar introselect), and the three order
+    the elements on either side of the kth differently, so the "median"
+    trees, whose leaf order is that permutation, hold for machines whose
+    numpy gives the same fingerprint."""
+    rng = np.random.default_rng(20261204)
+    h = hashlib.sha256()
+    for n in (9, 100, 1000, 100003):
+        a = rng.random(n).astype(np.float32)
+        h.update(np.argpartition(a, n // 2).astype(np.int64).tobytes())
+    return h.hexdigest()
+
+
 def digest(arrays):
     """SHA-256 over the arrays in a fixed field order (little-endian bytes)."""
     h = hashlib.sha256()
@@ -159,6 +176,7 @@ def main():
             out[f"{name}_{method}_sha256"] = np.asarray(digest(arrays))
             out[f"{name}_{method}_shape"] = np.asarray([len(v0), len(b.nodes)], np.int64)
             print(f"{name} {method}: {len(v0)} triangles, {len(b.nodes)} nodes", flush=True)
+    out["argpartition_fingerprint"] = np.asarray(argpartition_fingerprint())
     np.savez_compressed(ROOT / "tests" / "golden" / "bvh_builder.npz", **out)
 
 

@@ -45,10 +45,21 @@ def assert_tree_matches(arrays, g, prefix):
 BUILD = {"morton": bvh_oracle.build_morton, "median": bvh_oracle.build_median}
 
 
+def _median_comparable(g, method):
+    """The "median" trees hold where numpy's argpartition permutes as on
+    the machine that wrote them (its routine depends on the CPU's SIMD
+    features: the leaf order of rendering/taichi/bvh_builder.py:510 does
+    too, for the reference itself)."""
+    if method == "median" and gb.argpartition_fingerprint() != str(g["argpartition_fingerprint"]):
+        pytest.skip("numpy's argpartition on this CPU orders partitions differently from the machine that "
+                    "wrote the golden; the partition invariant is pinned in test_oracle_pins")
+
+
 @pytest.mark.parametrize("method", ["morton", "median"])
 @pytest.mark.parametrize("k", range(len(gb.CASES)))
 def test_bvh_oracle_matches_reference_builder(k, method):
     g = _bvh()
+    _median_comparable(g, method)
     n, seed = (int(x) for x in g["cases"][k])
     kind = str(g["kinds"][k])
     v0, v1, v2 = gb.mesh(n, seed, kind)
@@ -58,6 +69,7 @@ def test_bvh_oracle_matches_reference_builder(k, method):
 @pytest.mark.parametrize("method", ["morton", "median"])
 def test_bvh_oracle_matches_reference_builder_on_scene_meshes(method):
     g = _bvh()
+    _median_comparable(g, method)
     for name in ("cornell", "mesh"):
         v0, v1, v2 = gb.scene_triangles(name)
         t = BUILD[method](v0, v1, v2, 8)
