@@ -509,13 +509,14 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 active = true;
                 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
                 org = ld3(J.camera.pos);
-                w.thr_x[p] = 1.0f;
-                w.thr_y[p] = 1.0f;
-                w.thr_z[p] = 1.0f;
-                w.col_x[p] = 0.0f;
-                w.col_y[p] = 0.0f;
-                w.col_z[p] = 0.0f;
-                w.ior[p] = 1.0f;
+                // throughput, colour and IOR start as 1, 0 and 1: the
+                // camera-ray k_shade takes them as constants and writes them
+                // (a job of depth 0 is never shaded: its samples stay black)
+                if (J.params.max_depth <= 0) {
+                    w.col_x[p] = 0.0f;
+                    w.col_y[p] = 0.0f;
+                    w.col_z[p] = 0.0f;
+                }
                 w.fn_x[p] = 0.0f;
                 w.fn_y[p] = 0.0f;
                 w.fn_z[p] = 0.0f;
@@ -1103,8 +1104,9 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
             float t, hu, hv;
             int kind, id;
             get_hit(w, i, t, kind, id, hu, hv);
-            f3 thr = mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
-            f3 col = mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+            // a camera ray's path starts with throughput 1 and colour 0
+            f3 thr = depth == 0 ? mk(1.0f, 1.0f, 1.0f) : mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
+            f3 col = depth == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(w.col_x[p], w.col_y[p], w.col_z[p]);
             if (t < 0.0f || t >= 1e9f) {
                 // trace.py:154-159
                 f3 sky;
@@ -1173,7 +1175,7 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
                             nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
                         thr = thr * albedo;
                     } else if (m.type == RX_MAT_DIELECTRIC) {
-                        float cur = w.ior[p];
+                        float cur = depth == 0 ? 1.0f : w.ior[p];
                         float eta;
                         if (front) {
                             eta = cur / m.ior;
@@ -1205,6 +1207,9 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
                     w.thr_x[p] = thr.x;
                     w.thr_y[p] = thr.y;
                     w.thr_z[p] = thr.z;
+                    // the IOR a camera ray's path continues in (a dielectric
+                    // wrote its own above)
+                    if (depth == 0 && m.type != RX_MAT_DIELECTRIC) w.ior[p] = 1.0f;
                 }
             }
             w.col_x[p] = col.x;
