@@ -75,6 +75,10 @@ struct ws_view {
     // job, whose stop drops samples; otherwise the traversal kernels add
     // their wave totals to job 0's ray partials directly (set per pass)
     int ray_per_path;
+    // 1: the primary queue's slots hold no ray records (camera rays are
+    // traced from their pixel-major copy, and the camera-ray k_shade
+    // recomputes each slot's ray from its pixel and sample; set per pass)
+    int cam_rays;
     // two ray queues: 32 B per slot, {origin, path index bits} {direction, 0}
     gv4f* q_ray[2];
     // hits of the current queue: 16 B per slot, {t, coded id bits, u, v}
@@ -161,6 +165,7 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.p_job = ar.take<int32_t>(P);
     w.p_rays = ar.take<int32_t>(P);
     w.ray_per_path = 1;
+    w.cam_rays = 0;
     for (int b = 0; b < 2; ++b) w.q_ray[b] = ar.take<gv4f>(2 * P);
     w.h_rec = ar.take<gv4f>(P);
     w.e_rec = ar.take<gv4f>(4 * P);
@@ -528,7 +533,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 if (w.ray_per_path) w.p_rays[p] = 0;
             }
         }
-        if (active) {
+        if (active && !w.cam_rays) {
             int slot = w.q_off[jx] + (int)(p - w.path_off[jx]);
             put_ray(w, 0, slot, (int)p, org, rd);
         }
@@ -1091,14 +1096,27 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
         float scale = 1.0f;
         if (valid) {
             f3 ro, rd;
-            p = get_ray(w, qi, i, ro, rd);
-            int jx = job_of_path(w, p);
+            const bool cam = depth == 0 && w.cam_rays;
+            int jx;
+            if (cam) {
+                // primary slot i: path path_off + (i - q_off) of its job
+                jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
+                p = w.path_off[jx] + (i - w.q_off[jx]);
+            } else {
+                p = get_ray(w, qi, i, ro, rd);
+                jx = job_of_path(w, p);
+            }
             const rx_job& J = w.jobs[jx];
             const rx_scene& sc = J.scene;
             const rx_render_params& prm = J.params;
             int npix = J.camera.width * J.camera.height;
             int local = p - w.path_off[jx];
             int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
+            if (cam) {
+                // k_generate's camera ray of this pixel and sample
+                ro = ld3(J.camera.pos);
+                rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, frame);
+            }
             bool last = depth + 1 >= prm.max_depth;
             bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
             float t, hu, hv;
@@ -1890,6 +1908,9 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     // passes of several samples per pixel trace their primary rays
     // pixel-major (k_generate_pm); later bounces are sorted below
     const bool pm = pl.sort_rays && w.q_sorted && pl.max_spp > 1;
+    // traced from the pixel-major copy and shaded from the camera: the
+    // sample-major records would only be read by the primitive pre-pass
+    w.cam_rays = pm && !pl.prim_prepass ? 1 : 0;
     RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     if (pm) RX_STAGE(tm, 6, (k_generate_pm<<<gp, BS, 0, s>>>(w, w.q_sorted, w.si_in)));
     for (int d = 0; d < pl.max_depth; ++d) {
