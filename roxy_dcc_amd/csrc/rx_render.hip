@@ -529,7 +529,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 w.fa_y[p] = 0.0f;
                 w.fa_z[p] = 0.0f;
                 w.fd[p] = 0.0f;
-                w.p_job[p] = jx;
+                if (w.n_jobs > 1) w.p_job[p] = jx;  // job_of_path reads it only then
                 if (w.ray_per_path) w.p_rays[p] = 0;
             }
         }
