@@ -1771,7 +1771,40 @@ __device__ __forceinline__ uint32_t spread7(uint32_t v) {
     return v;
 }
 
-// ob: origin bits per axis (0..7)
+// The single-job key: a two-point key (Meister et al., "On Ray Reordering
+// Techniques for Faster GPU Ray Tracing", I3D 2020), the 6-D Morton code of
+// the origin and of the point half the scene's largest extent along the
+// ray, 5 bits per coordinate of the scene box (30 bits): rays that start
+// near each other and head the same way sort together.  Measured against
+// the direction-major key below: 2M-triangle scene 23.28-23.31 against
+// 23.94-23.96 ms per render, mesh 33.02-33.20 against 33.04-33.40 (a
+// quarter extent: 23.17 / 33.52, a whole extent: 23.57 / 33.00-33.07).
+constexpr float KEY_REACH = 0.5f;
+__device__ __forceinline__ uint32_t spread5_6d(uint32_t v) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) r |= ((v >> b) & 1u) << (6 * b);
+    return r;
+}
+__device__ __forceinline__ uint32_t ray_key_two_point(f3 o, f3 d, const rx_bvh_header& h) {
+    const float oc[3] = {o.x, o.y, o.z}, dc[3] = {d.x, d.y, d.z};
+    float ext = 0.0f;
+    for (int a = 0; a < 3; ++a) ext = fmaxf(ext, h.bmax[a] - h.bmin[a]);
+    uint32_t key = 0;
+    for (int a = 0; a < 3; ++a) {
+        const float span = fmaxf(h.bmax[a] - h.bmin[a], 1e-20f);
+        const float u0 = (oc[a] - h.bmin[a]) / span;
+        const float u1 = (oc[a] + dc[a] * (KEY_REACH * ext) - h.bmin[a]) / span;
+        const uint32_t q0 = (uint32_t)(fminf(fmaxf(u0, 0.0f), 1.0f) * 31.0f);
+        const uint32_t q1 = (uint32_t)(fminf(fmaxf(u1, 0.0f), 1.0f) * 31.0f);
+        key |= spread5_6d(q0) << (5 - a);
+        key |= spread5_6d(q1) << (2 - a);
+    }
+    return key;
+}
+
+// The batch key: the job, the direction quantised to 8 levels per axis,
+// then a Morton code of the origin with ob bits per axis (0..7).
 __device__ __forceinline__ uint32_t ray_key(f3 o, f3 d, const rx_bvh_header& h, uint32_t job, int ob) {
     uint32_t q[3];
     const float oc[3] = {o.x, o.y, o.z};
@@ -1796,7 +1829,9 @@ __global__ void k_ray_keys(ws_view w, int b, int ob, uint32_t* __restrict__ key,
         const int p = get_ray(w, b, (int)i, o, d);
         const int jx = job_of_path(w, p);
         const rx_scene& sc = w.jobs[jx].scene;
-        key[i] = sc.n_tris > 0 ? ray_key(o, d, gload(sc.header), (uint32_t)jx, ob) : (uint32_t)jx << (9 + 3 * ob);
+        key[i] = sc.n_tris == 0 ? (uint32_t)jx << (9 + 3 * ob)
+                 : w.n_jobs == 1 ? ray_key_two_point(o, d, gload(sc.header))
+                                 : ray_key(o, d, gload(sc.header), (uint32_t)jx, ob);
         idx[i] = (uint32_t)i;
     }
 }
