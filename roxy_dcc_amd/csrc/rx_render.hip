@@ -71,9 +71,9 @@ struct ws_view {
     float *fn_x, *fn_y, *fn_z, *fa_x, *fa_y, *fa_z, *fd;
     int32_t *p_job, *p_rays;
     // rays are counted per path (p_rays, summed by k_accumulate over the
-    // samples a job keeps) when a pass holds several jobs or an adaptive
-    // job, whose stop drops samples; otherwise the traversal kernels add
-    // their wave totals to job 0's ray partials directly (set per pass)
+    // samples a job keeps) when a pass holds an adaptive job, whose stop
+    // drops samples; otherwise the traversal kernels add their counts to
+    // the jobs' ray partials directly (lane_rays; set per pass)
     int ray_per_path;
     // 1: the primary queue's slots hold no ray records (camera rays are
     // traced from their pixel-major copy, and the camera-ray k_shade
@@ -496,6 +496,35 @@ __device__ __forceinline__ rx_counters* ctr_slot(const ws_view& w) {
     return reinterpret_cast<rx_counters*>(w.ctr_part + (blockIdx.x & (NPART - 1)) * 16);
 }
 
+// A lane's rays not yet added to its job's ray partials (passes without
+// per-path counts): added when the lane's next ray belongs to another job
+// (queues run job by job, so that is rare) and at the kernel's end, with
+// one atomic per wave when the wave's lanes share the job.
+struct lane_rays {
+    int job = 0;
+    unsigned long long n = 0;
+    __device__ __forceinline__ unsigned long long* slot(const ws_view& w, int j) const {
+        return w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs + j;
+    }
+    __device__ __forceinline__ void add(const ws_view& w, int j, unsigned long long k) {
+        if (j != job) {
+            if (n) atomicAdd(slot(w, job), n);
+            job = j;
+            n = 0;
+        }
+        n += k;
+    }
+    // every lane of the wave
+    __device__ __forceinline__ void flush(const ws_view& w) {
+        const int j0 = __builtin_amdgcn_readfirstlane(job);
+        if (__ballot(n != 0 && job != j0) == 0ull)
+            wave_add(slot(w, j0), n);
+        else if (n)
+            atomicAdd(slot(w, job), n);
+        n = 0;
+    }
+};
+
 // ---------------------------------------------------------------- generate
 __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     const int stride = gridDim.x * BS;
@@ -874,6 +903,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
     unsigned long long rays = 0;
+    lane_rays jr;
     bool exhausted = false;
     int i = -1;  // queue slot of this lane's ray, -1 when idle
     int p = 0;
@@ -894,6 +924,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 const int jx = job_of_path(w, p);
                 sc = &w.jobs[jx].scene;
                 rays++;
+                if (!w.ray_per_path) jr.add(w, jx, 1);
                 // primitive scan first (intersect.py:259-287), or its
                 // result from k_prim_scan
                 if (PRE) {
@@ -1004,7 +1035,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     }
     rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, rays);
-    if (!w.ray_per_path) wave_add(w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs, rays);
+    if (!w.ray_per_path) jr.flush(w);
     if (COUNT) {
         wave_add(&ctr->node_pops, cnt.pops);
         wave_add(&ctr->tri_tests, cnt.tris);
@@ -1305,6 +1336,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     const int lane = threadIdx.x & 63;
     trav_count cnt{0, 0, 0};
     unsigned long long shadow = 0;
+    lane_rays jr;
     bool exhausted = false;
     int i = -1;  // request of this lane, -1 when idle
     // request state
@@ -1453,7 +1485,10 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
                 w.col_x[p] = w.col_x[p] + add.x;
                 w.col_y[p] = w.col_y[p] + add.y;
                 w.col_z[p] = w.col_z[p] + add.z;
-                if (w.ray_per_path) w.p_rays[p] += cast;
+                if (w.ray_per_path)
+                    w.p_rays[p] += cast;
+                else
+                    jr.add(w, jcur, (unsigned long long)cast);
                 shadow += cast;
                 i = -1;
             }
@@ -1462,7 +1497,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(CONNECT_WAV
     rx_counters* ctr = ctr_slot(w);
     wave_add(&ctr->rays, shadow);
     wave_add(&ctr->shadow_rays, shadow);
-    if (!w.ray_per_path) wave_add(w.ray_part + (size_t)(blockIdx.x & (NPART - 1)) * w.n_jobs, shadow);
+    if (!w.ray_per_path) jr.flush(w);
     if (COUNT) {
         wave_add(&ctr->shadow_node_pops, cnt.pops);
         wave_add(&ctr->shadow_tri_tests, cnt.tris);
@@ -1936,7 +1971,7 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     int gc = DEEP ? ge : render_trav_grid(w0.P, BS_CONNECT, CONNECT_BLOCKS_PER_CU, pl.trav_per_cu);
     ws_view w = w0;
     w.tl_nodes = TLN > 0 ? pl.tl_nodes : nullptr;
-    w.ray_per_path = w.n_jobs > 1 || pl.adaptive ? 1 : 0;
+    w.ray_per_path = pl.adaptive ? 1 : 0;
     int gp = grid_for(w.P, BS, 16);
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     if (w.tl_nodes) RX_STAGE(tm, 0, (k_treelet<<<1, TLB, 0, s>>>(pl.tl_nodes, pl.tl_header, w)));
