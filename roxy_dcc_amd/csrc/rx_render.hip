@@ -55,6 +55,12 @@ constexpr int FETCH_STRIDE = 32;  // ints: 128 bytes between counters
 // unit, each term capped at 2^14) so the sum, and the stop decision, do not
 // depend on the order of the atomics; EPART partial slots per job.
 constexpr int EPART = 16;
+
+// Camera rays test only the analytic primitives their pixel's 8x8 tile can
+// see (k_prim_tiles): up to PRIM_TILE_MAX primitives, PRIM_TILE_WORDS
+// 64-bit masks per tile.
+constexpr int PRIM_TILE_WORDS = 4;
+constexpr int PRIM_TILE_MAX = 64 * PRIM_TILE_WORDS;
 constexpr double ERR_SCALE = 4194304.0;
 constexpr float ERR_CAP = 16384.0f;
 
@@ -136,6 +142,12 @@ struct ws_view {
     rx_tri* eye_tris;
     rx_bvh_header* eye_header;
     gv4f* eye_prims;
+    // with the copies, for 1..PRIM_TILE_MAX analytic primitives: per 8x8
+    // pixel tile of the camera (tile_cols per row), the primitives a camera
+    // ray through the tile can hit, as PRIM_TILE_WORDS 64-bit masks
+    // (k_prim_tiles); null otherwise
+    uint64_t* tile_prims;
+    int tile_cols;
 };
 
 void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
@@ -149,6 +161,8 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.eye_tris = nullptr;
     w.eye_header = nullptr;
     w.eye_prims = nullptr;
+    w.tile_prims = nullptr;
+    w.tile_cols = 0;
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
@@ -209,10 +223,12 @@ __device__ __forceinline__ int job_of_path(const ws_view& w, int p) { return w.n
 
 // Ray queue records: two 16-byte words per slot, so a ray is one pair of
 // wide loads or stores.
-__device__ __forceinline__ void put_ray(const ws_view& w, int b, int slot, int path, f3 o, f3 d) {
+// A ray record: {origin, path index bits}, {direction, tag bits}; the tag
+// of a camera ray is its pixel's 8x8 tile (camera_tile), 0 otherwise.
+__device__ __forceinline__ void put_ray(const ws_view& w, int b, int slot, int path, f3 o, f3 d, int tag = 0) {
     RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)slot;
     q[0] = gv4f{o.x, o.y, o.z, __int_as_float(path)};
-    q[1] = gv4f{d.x, d.y, d.z, 0.0f};
+    q[1] = gv4f{d.x, d.y, d.z, __int_as_float(tag)};
 }
 
 __device__ __forceinline__ int get_ray(const ws_view& w, int b, int slot, f3& o, f3& d) {
@@ -220,6 +236,16 @@ __device__ __forceinline__ int get_ray(const ws_view& w, int b, int slot, f3& o,
     gv4f a = q[0], c = q[1];
     o = mk(a.x, a.y, a.z);
     d = mk(c.x, c.y, c.z);
+    return __float_as_int(a.w);
+}
+
+// get_ray() that also returns the record's tag
+__device__ __forceinline__ int get_ray_tag(const ws_view& w, int b, int slot, f3& o, f3& d, int& tag) {
+    const RX_GLOBAL gv4f* q = (const RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)slot;
+    gv4f a = q[0], c = q[1];
+    o = mk(a.x, a.y, a.z);
+    d = mk(c.x, c.y, c.z);
+    tag = __float_as_int(c.w);
     return __float_as_int(a.w);
 }
 
@@ -259,6 +285,11 @@ __device__ __forceinline__ int pass_samples(const rx_job& j) {
 // scanline order.  Either way a bijection on [0, W*H): sampling is keyed
 // by the pixel, so images do not depend on the slot order.
 __device__ __forceinline__ bool tiled(const rx_camera& c) { return ((c.width | c.height) & 7) == 0; }
+
+// The 8x8 tile of pixel (px, py), row-major over the image's tiles.
+__device__ __forceinline__ int camera_tile(const rx_camera& c, int px, int py) {
+    return (py >> 3) * ((c.width + 7) >> 3) + (px >> 3);
+}
 
 __device__ __forceinline__ int slot_pixel(const rx_camera& c, int k) {
     if (!tiled(c)) return k;
@@ -531,7 +562,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     for (long long base = (long long)xcd_block() * BS; base < w.P; base += stride) {
         long long p = base + threadIdx.x;
         bool active = false;
-        int jx = 0;
+        int jx = 0, tag = 0;
         f3 org = mk(0.0f, 0.0f, 0.0f), rd = org;
         if (p < w.P) {
             jx = w.n_jobs == 1 ? 0 : owner(w.path_off, w.n_jobs, p);
@@ -543,6 +574,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 active = true;
                 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
                 org = ld3(J.camera.pos);
+                tag = camera_tile(J.camera, pix % J.camera.width, pix / J.camera.width);
                 // throughput, colour and IOR start as 1, 0 and 1: the
                 // camera-ray k_shade takes them as constants and writes them
                 // (a job of depth 0 is never shaded: its samples stay black)
@@ -564,7 +596,7 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
         }
         if (active && !w.cam_rays) {
             int slot = w.q_off[jx] + (int)(p - w.path_off[jx]);
-            put_ray(w, 0, slot, (int)p, org, rd);
+            put_ray(w, 0, slot, (int)p, org, rd, tag);
         }
     }
 }
@@ -591,7 +623,8 @@ __global__ __launch_bounds__(BS) void k_generate_pm(ws_view w, gv4f* __restrict_
         f3 org = ld3(J.camera.pos);
         RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)out + 2 * i;
         q[0] = gv4f{org.x, org.y, org.z, __int_as_float((int)(w.path_off[jx] + local))};
-        q[1] = gv4f{rd.x, rd.y, rd.z, 0.0f};
+        q[1] = gv4f{rd.x, rd.y, rd.z,
+                    __int_as_float(camera_tile(J.camera, pix % J.camera.width, pix / J.camera.width))};
         slot[i] = (uint32_t)(w.q_off[jx] + local);
     }
 }
@@ -757,16 +790,20 @@ __global__ __launch_bounds__(BS) void k_eye_prep(ws_view w) {
 
 // closest_prim() of a camera ray of job 0 from the eye-relative records
 // (scalar loads: every lane reads the same record).
+// With per-tile masks (tile_prims), only the primitives of the union of
+// the active lanes' tiles' masks are tested, in ascending index order: a
+// primitive outside a tile's mask misses every camera ray through the tile
+// (k_prim_tiles), so the closest hit and its tie-break are the full scan's.
 template <bool COUNT>
-__device__ __forceinline__ void closest_prim_eye(const ws_view& w, f3 ro, f3 rd, float& prim_t, int& prim_id,
-                                                 trav_count& cnt) {
+__device__ __forceinline__ void closest_prim_eye(const ws_view& w, f3 ro, f3 rd, int tile, float& prim_t,
+                                                 int& prim_id, trav_count& cnt) {
     const RX_CONST rx_job* J = (const RX_CONST rx_job*)(w.jobs);
     const int np = J->scene.n_prims;
     const rx_prim* prims = J->scene.prims;
     const RX_CONST gv4f* ep = (const RX_CONST gv4f*)w.eye_prims;
     prim_t = 1e9f;
     prim_id = -1;
-    for (int k = 0; k < np; ++k) {
+    auto test = [&](int k) {
         const gv4f r0 = ep[2 * k], r1 = ep[2 * k + 1];
         const float t = __float_as_int(r1.w) == EYE_PRIM_NONE ? prim_hit(cload(prims + k), ro, rd)
                                                               : prim_hit_rel(r0, r1, rd);
@@ -775,6 +812,120 @@ __device__ __forceinline__ void closest_prim_eye(const ws_view& w, f3 ro, f3 rd,
             prim_t = t;
             prim_id = k;
         }
+    };
+    if (w.tile_prims) {
+        // union over the distinct tiles of the active lanes (one or two: a
+        // refill takes consecutive slots of a pixel-major queue)
+        uint64_t m[PRIM_TILE_WORDS];
+        for (int q = 0; q < PRIM_TILE_WORDS; ++q) m[q] = 0ull;
+        bool pend = true;
+        for (;;) {
+            const unsigned long long b = __ballot(pend);
+            if (b == 0ull) break;
+            const int t = __builtin_amdgcn_readlane(tile, __builtin_ctzll(b));
+            const RX_CONST uint64_t* tm = (const RX_CONST uint64_t*)w.tile_prims + PRIM_TILE_WORDS * (long long)t;
+            for (int q = 0; q < PRIM_TILE_WORDS; ++q) m[q] |= tm[q];
+            pend = pend && tile != t;
+        }
+        for (int q = 0; q < PRIM_TILE_WORDS; ++q) {
+            for (uint64_t bits = m[q]; bits != 0ull; bits &= bits - 1ull) test(64 * q + __builtin_ctzll(bits));
+        }
+        return;
+    }
+    for (int k = 0; k < np; ++k) test(k);
+}
+
+// Per 8x8 tile of the (single) job's camera: the analytic primitives some
+// camera ray through the tile may hit.  The tile's rays leave the eye e
+// in directions camera_dir gives for pixel coordinates in [x0, x1] x
+// [y0, y1] (the jitter stays in [0, 1]): the pyramid spanned by its four
+// corner directions.  A sphere whose centre lies outside one of the
+// pyramid's side planes by more than its radius plus a margin, or a cube
+// whose corners all lie outside one by more than the margin, misses every
+// such ray, and its bit stays clear; the same for the plane through e
+// normal to the pyramid's mean direction (every ray of the tile has a
+// positive component along it), which the side planes alone do not give
+// for what lies behind the eye.  The margin, 1e-3 of the distance from
+// e plus 1e-3, is orders of magnitude above the rounding of camera_dir's
+// fp32 directions (~1e-7 relative) and of the fp32 sphere and slab tests at
+// that distance, so a cleared primitive's fp32 test reports no hit past
+// 0.001 for any of those rays.  Planes and triangle slots keep their bits.
+// Computed in fp64 from the fp32 camera and primitive records.
+__global__ __launch_bounds__(BS) void k_prim_tiles(ws_view w) {
+    const rx_job& J = w.jobs[0];
+    const rx_camera& cam = J.camera;
+    const rx_scene& sc = J.scene;
+    const int cols = w.tile_cols, rows = (cam.height + 7) >> 3;
+    const int np = sc.n_prims;
+    const double ex = cam.pos[0], ey = cam.pos[1], ez = cam.pos[2];
+    for (int t = blockIdx.x * BS + threadIdx.x; t < cols * rows; t += gridDim.x * BS) {
+        const int x0 = (t % cols) * 8, y0 = (t / cols) * 8;
+        const int x1 = min(x0 + 8, cam.width), y1 = min(y0 + 8, cam.height);
+        double dir[4][3];
+        const int cx[4] = {x0, x1, x1, x0}, cy[4] = {y0, y0, y1, y1};
+        for (int c = 0; c < 4; ++c) {
+            const double u = (double)cx[c] / (double)cam.width * 2.0 - 1.0;
+            const double v = 1.0 - (double)cy[c] / (double)cam.height * 2.0;
+            const double a = u * (double)cam.aspect * (double)cam.fov_tan, b = v * (double)cam.fov_tan;
+            for (int k = 0; k < 3; ++k)
+                dir[c][k] = (double)cam.fwd[k] + (double)cam.right[k] * a + (double)cam.up[k] * b;
+        }
+        double mid[3];
+        for (int k = 0; k < 3; ++k) mid[k] = dir[0][k] + dir[1][k] + dir[2][k] + dir[3][k];
+        // inward unit normals of the four side planes (through e)
+        double nrm[5][3];
+        bool ok[5];
+        for (int c = 0; c < 4; ++c) {
+            const double* p = dir[c];
+            const double* q = dir[(c + 1) & 3];
+            double n[3] = {p[1] * q[2] - p[2] * q[1], p[2] * q[0] - p[0] * q[2], p[0] * q[1] - p[1] * q[0]};
+            const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            ok[c] = len > 0.0;
+            const double sg = (n[0] * mid[0] + n[1] * mid[1] + n[2] * mid[2]) < 0.0 ? -1.0 : 1.0;
+            for (int k = 0; k < 3; ++k) nrm[c][k] = ok[c] ? sg * n[k] / len : 0.0;
+        }
+        // the fifth plane: through e, normal to the mean direction, used
+        // when every corner direction points to its positive side
+        {
+            const double len = sqrt(mid[0] * mid[0] + mid[1] * mid[1] + mid[2] * mid[2]);
+            bool front = len > 0.0;
+            for (int c = 0; c < 4 && front; ++c)
+                front = dir[c][0] * mid[0] + dir[c][1] * mid[1] + dir[c][2] * mid[2] > 0.0;
+            ok[4] = front;
+            for (int k = 0; k < 3; ++k) nrm[4][k] = front ? mid[k] / len : 0.0;
+        }
+        uint64_t m[PRIM_TILE_WORDS];
+        for (int q = 0; q < PRIM_TILE_WORDS; ++q) m[q] = 0ull;
+        for (int k = 0; k < np && k < PRIM_TILE_MAX; ++k) {
+            const rx_prim pr = gload(sc.prims + k);
+            bool keep = true;
+            if (pr.type == RX_PRIM_SPHERE) {
+                const double c[3] = {(double)pr.a[0] - ex, (double)pr.a[1] - ey, (double)pr.a[2] - ez};
+                const double r = fabs((double)pr.a[3]);
+                const double margin = 1e-3 * (sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) + r) + 1e-3;
+                for (int f = 0; f < 5 && keep; ++f)
+                    if (ok[f] && nrm[f][0] * c[0] + nrm[f][1] * c[1] + nrm[f][2] * c[2] < -(r + margin)) keep = false;
+            } else if (pr.type == RX_PRIM_CUBE) {
+                double lo[3], hi[3], far2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const double l = (double)pr.a[a] - fabs((double)pr.b[a]), h = (double)pr.a[a] + fabs((double)pr.b[a]);
+                    const double e = a == 0 ? ex : (a == 1 ? ey : ez);
+                    lo[a] = l - e;
+                    hi[a] = h - e;
+                    const double fa = fmax(fabs(lo[a]), fabs(hi[a]));
+                    far2 += fa * fa;
+                }
+                const double margin = 1e-3 * sqrt(far2) + 1e-3;
+                for (int f = 0; f < 5 && keep; ++f) {
+                    if (!ok[f]) continue;
+                    double most = 0.0;  // the corner furthest along the normal
+                    for (int a = 0; a < 3; ++a) most += fmax(nrm[f][a] * lo[a], nrm[f][a] * hi[a]);
+                    if (most < -margin) keep = false;
+                }
+            }
+            if (keep) m[k >> 6] |= 1ull << (k & 63);
+        }
+        for (int q = 0; q < PRIM_TILE_WORDS; ++q) w.tile_prims[(size_t)PRIM_TILE_WORDS * t + q] = m[q];
     }
 }
 
@@ -920,7 +1071,9 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
         {
             if (idle && mine < n) {
                 i = mine;
-                p = get_ray(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd);
+                int tag = 0;
+                p = EYE ? get_ray_tag(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd, tag)
+                        : get_ray(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd);
                 const int jx = job_of_path(w, p);
                 sc = &w.jobs[jx].scene;
                 rays++;
@@ -932,7 +1085,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     prim_t = h.x;
                     prim_id = __float_as_int(h.y);
                 } else if (EYE) {
-                    closest_prim_eye<COUNT>(w, ro, rd, prim_t, prim_id, cnt);
+                    closest_prim_eye<COUNT>(w, ro, rd, tag, prim_t, prim_id, cnt);
                 } else {
                     closest_prim_lanes<COUNT>(w, jx, ro, rd, prim_t, prim_id, cnt);
                 }
@@ -2171,17 +2324,25 @@ int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long lo
 // The eye-relative copies of a scene (k_eye_prep), after a render's own
 // workspace: the root record, the nodes, the triangles, 32 bytes per
 // primitive.
-void take_eye(rx_arena& ar, const rx_scene& sc, ws_view& w) {
+// ... and, for 1..PRIM_TILE_MAX primitives, the per-tile primitive masks
+// of a width x height camera.
+void take_eye(rx_arena& ar, const rx_scene& sc, int width, int height, ws_view& w) {
     w.eye_header = ar.take<rx_bvh_header>(1);
     w.eye_nodes = ar.take<rx_node>(sc.n_tris > 0 ? (size_t)sc.n_nodes : 0);
     w.eye_tris = ar.take<rx_tri>((size_t)sc.n_tris);
     w.eye_prims = ar.take<gv4f>(2 * (size_t)sc.n_prims);
+    w.tile_prims = nullptr;
+    w.tile_cols = (width + 7) >> 3;
+    if (sc.n_prims >= 1 && sc.n_prims <= PRIM_TILE_MAX) {
+        const size_t tiles = (size_t)w.tile_cols * (size_t)((height + 7) >> 3);
+        w.tile_prims = ar.take<uint64_t>(PRIM_TILE_WORDS * tiles);
+    }
 }
 
-size_t eye_bytes_for(const rx_scene& sc) {
+size_t eye_bytes_for(const rx_scene& sc, int width, int height) {
     rx_arena ar{nullptr, 0, 0};
     ws_view w;
-    take_eye(ar, sc, w);
+    take_eye(ar, sc, width, height, w);
     return ar.off + 256;
 }
 
@@ -2195,10 +2356,11 @@ bool eye_worthwhile(const rx_scene& sc, long long camera_rays) {
 
 // Carves the copies from the workspace past its first `used` bytes; false
 // (and no copies) when they do not fit.
-bool carve_eye(void* d_workspace, size_t bytes, size_t used, const rx_scene& sc, ws_view& w) {
-    if (bytes < used || bytes - used < eye_bytes_for(sc)) return false;
+bool carve_eye(void* d_workspace, size_t bytes, size_t used, const rx_scene& sc, int width, int height,
+               ws_view& w) {
+    if (bytes < used || bytes - used < eye_bytes_for(sc, width, height)) return false;
     rx_arena ar{reinterpret_cast<char*>(d_workspace), bytes, used};
-    take_eye(ar, sc, w);
+    take_eye(ar, sc, width, height, w);
     return true;
 }
 
@@ -2257,7 +2419,7 @@ extern "C" int rx_render_workspace_size_for_scene(const rx_scene* scene, int wid
     const int depth = scene->n_tris > 0 ? scene->bvh_depth : 0;
     const int st = rx_render_workspace_size(width, height, samples_per_pass, depth, bytes);
     if (st) return st;
-    *bytes += eye_bytes_for(*scene);
+    *bytes += eye_bytes_for(*scene, width, height);
     return RX_OK;
 }
 
@@ -2291,7 +2453,7 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
     const long long camera_rays = (long long)camera->width * camera->height * n_samples;
     const bool eye = eye_worthwhile(j.scene, camera_rays) &&
                      carve_eye(d_workspace, workspace_bytes, workspace_bytes_for(1, P, pl.deep, pl.max_spp, true),
-                               j.scene, w);
+                               j.scene, camera->width, camera->height, w);
     hipStream_t s = rx_stream(stream);
     k_set_job<<<1, 1, 0, s>>>(w.jobs, j, w.path_off, w.pix_off, poff[1], qoff[1]);
     RX_CHECK_LAUNCH();
@@ -2303,6 +2465,11 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
         if (j.scene.n_prims > recs) recs = j.scene.n_prims;
         k_eye_prep<<<grid_for(recs > 0 ? recs : 1, BS, 16), BS, 0, s>>>(w);
         RX_CHECK_LAUNCH();
+        if (w.tile_prims) {
+            const long long tiles = (long long)w.tile_cols * ((camera->height + 7) >> 3);
+            k_prim_tiles<<<grid_for(tiles, BS, 16), BS, 0, s>>>(w);
+            RX_CHECK_LAUNCH();
+        }
     }
     stage_timer tm;
     tm.on = (params->flags & RX_FLAG_PROFILE) != 0;

@@ -34,17 +34,21 @@ def test_library_exports_every_declared_symbol():
 def test_scene_workspace_holds_the_eye_relative_copies():
     """rx_render_workspace_size_for_scene = the render's own workspace plus
     the eye-relative copies: 64 bytes per node, 48 per triangle, 32 per
-    primitive and the root record (each array 256-byte aligned)."""
+    primitive and the root record, and for 1..256 primitives 32 bytes of
+    primitive masks per 8x8 tile of the camera (each array 256-byte
+    aligned)."""
     lib = nat.lib()
-    for n_nodes, n_tris, n_prims, depth in ((1000, 2000, 5, 10), (0, 0, 67, 0), (524287, 2_000_000, 67, 18)):
+    for n_nodes, n_tris, n_prims, depth in ((1000, 2000, 5, 10), (0, 0, 67, 0), (524287, 2_000_000, 67, 18),
+                                            (1000, 2000, 0, 10), (1000, 2000, 300, 10)):
         sc = nat.RxScene()
         sc.n_nodes, sc.n_tris, sc.n_prims, sc.bvh_depth = n_nodes, n_tris, n_prims, depth
         a, b = C.c_size_t(), C.c_size_t()
-        assert lib.rx_render_workspace_size_for_scene(C.byref(sc), 256, 128, 4, C.byref(a)) == 0
-        assert lib.rx_render_workspace_size(256, 128, 4, depth, C.byref(b)) == 0
+        assert lib.rx_render_workspace_size_for_scene(C.byref(sc), 250, 128, 4, C.byref(a)) == 0
+        assert lib.rx_render_workspace_size(250, 128, 4, depth, C.byref(b)) == 0
         extra = a.value - b.value
-        payload = 64 * n_nodes + 48 * n_tris + 32 * n_prims + 32
-        assert payload <= extra <= payload + 5 * 256, (extra, payload)
+        tiles = 32 * 16 if 1 <= n_prims <= 256 else 0
+        payload = 64 * n_nodes + 48 * n_tris + 32 * n_prims + 32 + 32 * tiles
+        assert payload <= extra <= payload + 6 * 256, (extra, payload)
     assert lib.rx_render_workspace_size_for_scene(None, 256, 128, 4, C.byref(a)) == nat.RX_ERR_INVALID
 
 
