@@ -374,10 +374,13 @@ int rx_render_workspace_size(int width, int height, int samples_per_pass,
  * scene that rx_render's camera-ray traversal reads when they fit: node
  * boxes and triangle vertices minus the camera position, with the terms of
  * the intersection tests built from them alone (64 bytes per node, 48 per
- * triangle, 32 per analytic primitive).  Same images and counters either
- * way; a workspace of rx_render_workspace_size() bytes renders without
- * them.  No counterpart in the reference (render_kernel recomputes these
- * per ray, rendering/taichi/intersect.py:28-78). */
+ * triangle, 32 per analytic primitive), and with 1..256 analytic
+ * primitives 32 bytes per 8x8 pixel tile: the primitives the tile's camera
+ * rays can hit, the only ones they then test.  Same images, ray counts,
+ * node pops and triangle tests either way (primitive tests can only drop);
+ * a workspace of rx_render_workspace_size() bytes renders without them.
+ * No counterpart in the reference (render_kernel recomputes these per ray
+ * and scans every primitive, rendering/taichi/intersect.py:28-78,259-287). */
 int rx_render_workspace_size_for_scene(const rx_scene* scene, int width, int height,
                                        int samples_per_pass, size_t* bytes);
 int rx_render(const rx_scene* scene, const rx_camera* camera,
