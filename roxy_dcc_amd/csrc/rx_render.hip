@@ -1253,13 +1253,148 @@ __device__ __forceinline__ f3 dielectric_dir(f3 d, f3 n, float eta, float xi) {
 // larger blocks mean fewer same-address atomics (mesh: 4.28 ms of shading
 // per 64-spp render with 256 threads, 3.28 with 512, 3.38 with 1024).
 constexpr int SHADE_BS = 512;
-// 6 waves per SIMD: 76 VGPRs and no spills (left to itself the compiler
-// takes 116 VGPRs, 4 waves; at 8 waves it spills).  Shading is a stream
-// of dependent loads and stores, so the extra waves keep more of them in
-// flight (mesh: 3.08 against 3.28 ms of shading per render).
+// 8 waves per SIMD: 63 VGPRs and no VGPR spills with the shading in
+// shade_hit (left to itself the compiler takes 116 VGPRs, 4 waves).
+// Shading is a stream of dependent loads and stores, so the extra waves
+// keep more of them in flight (mesh: 3.28 ms of shading per render at 4
+// waves, 3.08 at 6, 2.96 at 8; 2M-triangle scene 1.22 -> 1.13 ms from 6
+// to 8).
 #ifndef RX_SHADE_WAVES_PER_EU
-#define RX_SHADE_WAVES_PER_EU 6
+#define RX_SHADE_WAVES_PER_EU 8
 #endif
+// What shading one hit hands on: the continuation ray and the next-event
+// request (cont / want_nee say whether there is one).
+struct shade_out {
+    bool cont, want_nee;
+    f3 ro2, rd2, hit_p, nrm, alb, thr0;
+    float scale;
+};
+// rendering/taichi/trace.py:154-234 for path p at bounce `depth`, whose ray
+// (ro, rd) hit (t, kind, id, hu, hv) (a miss: t < 0 or t >= 1e9): sky or
+// background on a miss, emission, or the material's next-event request
+// and scattered ray with Russian roulette past depth 2.  thr and col are
+// the path's throughput and colour, updated in place (the caller stores
+// col); the throughput and the current IOR are stored in the path's
+// slots, and at depth 0 the first-hit normal, albedo and depth.
+__device__ __forceinline__ void shade_hit(const ws_view& w, const rx_job& J, int p, int pix, int frame, int depth,
+                                          f3 ro, f3 rd, float t, int kind, int id, float hu, float hv, f3& thr,
+                                          f3& col, shade_out& o) {
+    const rx_scene& sc = J.scene;
+    const rx_render_params& prm = J.params;
+    const bool last = depth + 1 >= prm.max_depth;
+    const bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
+    o.cont = false;
+    o.want_nee = false;
+    o.scale = 1.0f;
+    o.hit_p = o.nrm = o.alb = o.thr0 = o.ro2 = o.rd2 = mk(0.0f, 0.0f, 0.0f);
+    if (t < 0.0f || t >= 1e9f) {
+        // trace.py:154-159
+        f3 sky;
+        if (prm.use_sky) {
+            float s = 0.5f * (rd.y + 1.0f);
+            sky = mk(1.0f, 1.0f, 1.0f) * (1.0f - s) + mk(0.5f, 0.7f, 1.0f) * s;
+        } else {
+            sky = ld3(prm.bg);
+        }
+        col = col + thr * sky;
+        return;
+    }
+    rx_material m;
+    f3 nn, albedo;
+    if (kind == RX_KIND_BVH) {
+        const rx_tri_shade sh = gload(sc.tri_shade + id);
+        float u = hu, v = hv;
+        float ww = (1.0f - u) - v;
+        nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
+        m = gload(sc.materials + sh.mat);
+        albedo = bvh_albedo(sc, m, id, sh.has_uv, u, v, ld3(m.albedo));
+    } else {
+        const rx_prim pr = gload(sc.prims + id);
+        nn = prim_normal(pr, ro, rd, t);
+        m = gload(sc.prim_materials + id);
+        albedo = ld3(m.albedo);
+    }
+    // trace.py:161-165
+    bool front = dot(nn, rd) < 0.0f;
+    if (!front) nn = -nn;
+    const f3 hit_p = ro + rd * t;
+    o.hit_p = hit_p;
+    if (depth == 0) {
+        w.fn_x[p] = nn.x;
+        w.fn_y[p] = nn.y;
+        w.fn_z[p] = nn.z;
+        w.fa_x[p] = albedo.x;
+        w.fa_y[p] = albedo.y;
+        w.fa_z[p] = albedo.z;
+        w.fd[p] = t;
+    }
+    if (m.type == RX_MAT_EMISSIVE) {
+        col = col + (thr * albedo) * m.emission;
+        return;
+    }
+    u4 r0 = rng(prm, pix, frame, depth, 0u);
+    u4 r1 = rng(prm, pix, frame, depth, 1u);
+    f3 nd = rd;
+    if (m.type == RX_MAT_DIFFUSE) {
+        if (nee_on) {
+            o.want_nee = true;
+            o.scale = 1.0f;
+        }
+        o.thr0 = thr;
+        nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
+        thr = thr * albedo;
+    } else if (m.type == RX_MAT_METAL) {
+        nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
+        thr = thr * albedo;
+    } else if (m.type == RX_MAT_GLOSSY) {
+        if (nee_on) {
+            o.want_nee = true;
+            o.scale = m.roughness;
+        }
+        o.thr0 = thr;
+        if (u01(r0.w) < m.roughness)
+            nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
+        else
+            nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
+        thr = thr * albedo;
+    } else if (m.type == RX_MAT_DIELECTRIC) {
+        float cur = depth == 0 ? 1.0f : w.ior[p];
+        float eta;
+        if (front) {
+            eta = cur / m.ior;
+            cur = m.ior;
+        } else {
+            eta = cur / 1.0f;
+            cur = 1.0f;
+        }
+        w.ior[p] = cur;
+        nd = dielectric_dir(rd, nn, eta, u01(r0.w));
+        thr = thr * albedo;
+    }
+    o.nrm = nn;
+    o.alb = albedo;
+    // trace.py:227-234
+    bool alive = true;
+    if (depth > 2) {
+        float rr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+        if (u01(r1.z) > rr)
+            alive = false;
+        else
+            thr = thr / rr;
+    }
+    if (alive && !last) {
+        o.cont = true;
+        o.ro2 = hit_p + nn * 1e-4f;
+        o.rd2 = nd;
+    }
+    w.thr_x[p] = thr.x;
+    w.thr_y[p] = thr.y;
+    w.thr_z[p] = thr.z;
+    // the IOR a camera ray's path continues in (a dielectric wrote its own
+    // above)
+    if (depth == 0 && m.type != RX_MAT_DIELECTRIC) w.ior[p] = 1.0f;
+}
+
 __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHADE_WAVES_PER_EU))) void k_shade(
     ws_view w, int qi, int depth) {
     constexpr int BS = SHADE_BS;
@@ -1274,10 +1409,9 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
     for (int base = xcd_block() * BS; base < n; base += stride) {
         int i = base + threadIdx.x;
         bool valid = i < n;
-        bool cont = false, want_nee = false;
         int p = 0;
-        f3 hit_p = mk(0.0f, 0.0f, 0.0f), nrm = hit_p, alb = hit_p, thr0 = hit_p, ro2 = hit_p, rd2 = hit_p;
-        float scale = 1.0f;
+        shade_out o;
+        o.cont = o.want_nee = false;
         if (valid) {
             f3 ro, rd;
             const bool cam = depth == 0 && w.cam_rays;
@@ -1291,8 +1425,6 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
                 jx = job_of_path(w, p);
             }
             const rx_job& J = w.jobs[jx];
-            const rx_scene& sc = J.scene;
-            const rx_render_params& prm = J.params;
             int npix = J.camera.width * J.camera.height;
             int local = p - w.path_off[jx];
             int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
@@ -1301,134 +1433,28 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
                 ro = ld3(J.camera.pos);
                 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, frame);
             }
-            bool last = depth + 1 >= prm.max_depth;
-            bool nee_on = depth < prm.direct_light_max_depth && sc.n_lights > 0;
             float t, hu, hv;
             int kind, id;
             get_hit(w, i, t, kind, id, hu, hv);
             // a camera ray's path starts with throughput 1 and colour 0
             f3 thr = depth == 0 ? mk(1.0f, 1.0f, 1.0f) : mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
             f3 col = depth == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(w.col_x[p], w.col_y[p], w.col_z[p]);
-            if (t < 0.0f || t >= 1e9f) {
-                // trace.py:154-159
-                f3 sky;
-                if (prm.use_sky) {
-                    float s = 0.5f * (rd.y + 1.0f);
-                    sky = mk(1.0f, 1.0f, 1.0f) * (1.0f - s) + mk(0.5f, 0.7f, 1.0f) * s;
-                } else {
-                    sky = ld3(prm.bg);
-                }
-                col = col + thr * sky;
-            } else {
-                rx_material m;
-                f3 nn, albedo;
-                if (kind == RX_KIND_BVH) {
-                    const rx_tri_shade sh = gload(sc.tri_shade + id);
-                    float u = hu, v = hv;
-                    float ww = (1.0f - u) - v;
-                    nn = normalize((ld3(sh.n0) * ww + ld3(sh.n1) * u) + ld3(sh.n2) * v);
-                    m = gload(sc.materials + sh.mat);
-                    albedo = bvh_albedo(sc, m, id, sh.has_uv, u, v, ld3(m.albedo));
-                } else {
-                    const rx_prim pr = gload(sc.prims + id);
-                    nn = prim_normal(pr, ro, rd, t);
-                    m = gload(sc.prim_materials + id);
-                    albedo = ld3(m.albedo);
-                }
-                // trace.py:161-165
-                bool front = dot(nn, rd) < 0.0f;
-                if (!front) nn = -nn;
-                hit_p = ro + rd * t;
-                if (depth == 0) {
-                    w.fn_x[p] = nn.x;
-                    w.fn_y[p] = nn.y;
-                    w.fn_z[p] = nn.z;
-                    w.fa_x[p] = albedo.x;
-                    w.fa_y[p] = albedo.y;
-                    w.fa_z[p] = albedo.z;
-                    w.fd[p] = t;
-                }
-                if (m.type == RX_MAT_EMISSIVE) {
-                    col = col + (thr * albedo) * m.emission;
-                } else {
-                    u4 r0 = rng(prm, pix, frame, depth, 0u);
-                    u4 r1 = rng(prm, pix, frame, depth, 1u);
-                    f3 nd = rd;
-                    if (m.type == RX_MAT_DIFFUSE) {
-                        if (nee_on) {
-                            want_nee = true;
-                            scale = 1.0f;
-                        }
-                        thr0 = thr;
-                        nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
-                        thr = thr * albedo;
-                    } else if (m.type == RX_MAT_METAL) {
-                        nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
-                        thr = thr * albedo;
-                    } else if (m.type == RX_MAT_GLOSSY) {
-                        if (nee_on) {
-                            want_nee = true;
-                            scale = m.roughness;
-                        }
-                        thr0 = thr;
-                        if (u01(r0.w) < m.roughness)
-                            nd = diffuse_dir(nn, u01(r1.x), u01(r1.y));
-                        else
-                            nd = metal_dir(rd, nn, m.roughness, u01(r1.x), u01(r1.y));
-                        thr = thr * albedo;
-                    } else if (m.type == RX_MAT_DIELECTRIC) {
-                        float cur = depth == 0 ? 1.0f : w.ior[p];
-                        float eta;
-                        if (front) {
-                            eta = cur / m.ior;
-                            cur = m.ior;
-                        } else {
-                            eta = cur / 1.0f;
-                            cur = 1.0f;
-                        }
-                        w.ior[p] = cur;
-                        nd = dielectric_dir(rd, nn, eta, u01(r0.w));
-                        thr = thr * albedo;
-                    }
-                    nrm = nn;
-                    alb = albedo;
-                    // trace.py:227-234
-                    bool alive = true;
-                    if (depth > 2) {
-                        float rr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
-                        if (u01(r1.z) > rr)
-                            alive = false;
-                        else
-                            thr = thr / rr;
-                    }
-                    if (alive && !last) {
-                        cont = true;
-                        ro2 = hit_p + nn * 1e-4f;
-                        rd2 = nd;
-                    }
-                    w.thr_x[p] = thr.x;
-                    w.thr_y[p] = thr.y;
-                    w.thr_z[p] = thr.z;
-                    // the IOR a camera ray's path continues in (a dielectric
-                    // wrote its own above)
-                    if (depth == 0 && m.type != RX_MAT_DIELECTRIC) w.ior[p] = 1.0f;
-                }
-            }
+            shade_hit(w, J, p, pix, frame, depth, ro, rd, t, kind, id, hu, hv, thr, col, o);
             w.col_x[p] = col.x;
             w.col_y[p] = col.y;
             w.col_z[p] = col.z;
         }
-        int slot = block_append<BS>(&w.counts[qo], cont, app);
-        if (cont) {
-            put_ray(w, qo, slot, p, ro2, rd2);
+        int slot = block_append<BS>(&w.counts[qo], o.cont, app);
+        if (o.cont) {
+            put_ray(w, qo, slot, p, o.ro2, o.rd2);
         }
-        int es = block_append<BS>(&w.counts[2], want_nee, app);
-        if (want_nee) {
+        int es = block_append<BS>(&w.counts[2], o.want_nee, app);
+        if (o.want_nee) {
             RX_GLOBAL gv4f* e = (RX_GLOBAL gv4f*)w.e_rec + 4 * (long long)es;
-            e[0] = gv4f{hit_p.x, hit_p.y, hit_p.z, __int_as_float(p)};
-            e[1] = gv4f{nrm.x, nrm.y, nrm.z, scale};
-            e[2] = gv4f{alb.x, alb.y, alb.z, 0.0f};
-            e[3] = gv4f{thr0.x, thr0.y, thr0.z, 0.0f};
+            e[0] = gv4f{o.hit_p.x, o.hit_p.y, o.hit_p.z, __int_as_float(p)};
+            e[1] = gv4f{o.nrm.x, o.nrm.y, o.nrm.z, o.scale};
+            e[2] = gv4f{o.alb.x, o.alb.y, o.alb.z, 0.0f};
+            e[3] = gv4f{o.thr0.x, o.thr0.y, o.thr0.z, 0.0f};
         }
     }
 }
