@@ -576,20 +576,15 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
                 org = ld3(J.camera.pos);
                 tag = camera_tile(J.camera, pix % J.camera.width, pix / J.camera.width);
                 // throughput, colour and IOR start as 1, 0 and 1: the
-                // camera-ray k_shade takes them as constants and writes them
-                // (a job of depth 0 is never shaded: its samples stay black)
+                // camera-ray k_shade takes them as constants and writes them,
+                // and the first-hit depth (0 for a miss).  A job of depth 0
+                // is never shaded: its samples stay black with no first hit.
                 if (J.params.max_depth <= 0) {
                     w.col_x[p] = 0.0f;
                     w.col_y[p] = 0.0f;
                     w.col_z[p] = 0.0f;
+                    w.fd[p] = 0.0f;
                 }
-                w.fn_x[p] = 0.0f;
-                w.fn_y[p] = 0.0f;
-                w.fn_z[p] = 0.0f;
-                w.fa_x[p] = 0.0f;
-                w.fa_y[p] = 0.0f;
-                w.fa_z[p] = 0.0f;
-                w.fd[p] = 0.0f;
                 if (w.n_jobs > 1) w.p_job[p] = jx;  // job_of_path reads it only then
                 if (w.ray_per_path) w.p_rays[p] = 0;
             }
@@ -1297,6 +1292,8 @@ __device__ __forceinline__ void shade_hit(const ws_view& w, const rx_job& J, int
             sky = ld3(prm.bg);
         }
         col = col + thr * sky;
+        // no first hit: depth 0, and k_accumulate takes normal and albedo 0
+        if (depth == 0) w.fd[p] = 0.0f;
         return;
     }
     rx_material m;
@@ -1817,9 +1814,15 @@ __global__ __launch_bounds__(BS) void k_accumulate(ws_view w) {
                     long long p = (long long)w.path_off[jx] + (long long)s * npix + k;
                     int frame = J.first_sample + s;
                     f3 smp = clamped_sample(w, prm, p);
-                    f3 ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
-                    f3 as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
+                    // a path's first-hit normal and albedo are written only
+                    // when its camera ray hit something (depth > 0.001);
+                    // otherwise they are 0
                     float ds = w.fd[p];
+                    f3 ns_ = mk(0.0f, 0.0f, 0.0f), as_ = ns_;
+                    if (ds > 0.0f) {
+                        ns_ = mk(w.fn_x[p], w.fn_y[p], w.fn_z[p]);
+                        as_ = mk(w.fa_x[p], w.fa_y[p], w.fa_z[p]);
+                    }
                     if (frame == 0) {
                         c = smp;
                         nn = ns_;
@@ -2160,7 +2163,11 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     // traced from the pixel-major copy and shaded from the camera: the
     // sample-major records would only be read by the primitive pre-pass
     w.cam_rays = pm && !pl.prim_prepass ? 1 : 0;
-    RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
+    // k_generate has nothing to write for a single job whose camera rays
+    // are traced from the pixel-major copy and shaded from the camera, with
+    // rays counted per wave and every path shaded at depth 0
+    if (!(w.n_jobs == 1 && w.cam_rays && !w.ray_per_path && pl.max_depth >= 1))
+        RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
     if (pm) RX_STAGE(tm, 6, (k_generate_pm<<<gp, BS, 0, s>>>(w, w.q_sorted, w.si_in)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
