@@ -221,10 +221,10 @@ __device__ __forceinline__ int lanes_below(unsigned long long m) {
 
 __device__ __forceinline__ int job_of_path(const ws_view& w, int p) { return w.n_jobs == 1 ? 0 : w.p_job[p]; }
 
-// Ray queue records: two 16-byte words per slot, so a ray is one pair of
-// wide loads or stores.
-// A ray record: {origin, path index bits}, {direction, tag bits}; the tag
-// of a camera ray is its pixel's 8x8 tile (camera_tile), 0 otherwise.
+// Ray queue records: two 16-byte words per slot, {origin, path index
+// bits} and {direction, tag bits}, so a ray is one pair of wide loads or
+// stores; the tag of a camera ray is its pixel's 8x8 tile (camera_tile),
+// 0 otherwise.
 __device__ __forceinline__ void put_ray(const ws_view& w, int b, int slot, int path, f3 o, f3 d, int tag = 0) {
     RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)w.q_ray[b] + 2 * (long long)slot;
     q[0] = gv4f{o.x, o.y, o.z, __int_as_float(path)};
