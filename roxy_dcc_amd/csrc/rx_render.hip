@@ -82,9 +82,13 @@ struct ws_view {
     // the jobs' ray partials directly (lane_rays; set per pass)
     int ray_per_path;
     // 1: the primary queue's slots hold no ray records (camera rays are
-    // traced from their pixel-major copy, and the camera-ray k_shade
-    // recomputes each slot's ray from its pixel and sample; set per pass)
+    // traced pixel-major, and the camera-ray k_shade recomputes each slot's
+    // ray from its pixel and sample; set per pass)
     int cam_rays;
+    // 1 for the pixel-major camera-ray launch of k_extend: its position i
+    // is a pixel and sample (pm_camera_ray), whose ray the kernel computes
+    // instead of reading a record (set per launch)
+    int cam_gen;
     // two ray queues: 32 B per slot, {origin, path index bits} {direction, 0}
     gv4f* q_ray[2];
     // hits of the current queue: 16 B per slot, {t, coded id bits, u, v}
@@ -111,13 +115,11 @@ struct ws_view {
     int err_stride;
     // traversal stack spill area (entries past the LDS part)
     int2* spill;
-    // reordering of the ray queues for k_extend: the pixel-major copy of the
-    // primary queue, the sort keys and slot indices (double buffers) and
-    // the radix sort's digit counts (rx_sort.h); hit_slot (set per launch)
-    // maps a position of the traced order back to the queue slot its hit
-    // belongs to, ray_slot (if set) to the slot its ray is read from (null:
-    // position i is slot i)
-    gv4f* q_sorted;
+    // reordering of the ray queues for k_extend: the sort keys and slot
+    // indices (double buffers) and the radix sort's digit counts
+    // (rx_sort.h); hit_slot (set per launch) maps a position of the traced
+    // order back to the queue slot its hit belongs to, ray_slot (if set) to
+    // the slot its ray is read from (null: position i is slot i)
     uint32_t *sk_in, *sk_out, *si_in, *si_out;
     uint32_t* sort_scratch;
     const uint32_t* hit_slot;
@@ -180,17 +182,16 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.p_rays = ar.take<int32_t>(P);
     w.ray_per_path = 1;
     w.cam_rays = 0;
+    w.cam_gen = 0;
     for (int b = 0; b < 2; ++b) w.q_ray[b] = ar.take<gv4f>(2 * P);
     w.h_rec = ar.take<gv4f>(P);
     w.e_rec = ar.take<gv4f>(4 * P);
     w.hit_slot = nullptr;
     w.ray_slot = nullptr;
     w.nee_slot = nullptr;
-    w.q_sorted = nullptr;
     w.sk_in = w.sk_out = w.si_in = w.si_out = nullptr;
     w.sort_scratch = nullptr;
     if (sort) {
-        w.q_sorted = ar.take<gv4f>(2 * P);
         w.sk_in = ar.take<uint32_t>(P);
         w.sk_out = ar.take<uint32_t>(P);
         w.si_in = ar.take<uint32_t>(P);
@@ -596,32 +597,28 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
     }
 }
 
-// The primary rays once more, pixel-major within each job: position
-// q_off[j] + k * ns + s of the queue holds sample s of pixel slot k (ns
+// Camera ray at position i of a pixel-major camera-ray launch: within
+// each job, position q_off[j] + k * ns + s is sample s of pixel slot k (ns
 // samples of the job this pass), so a wave traces the samples of one or a
 // few pixels — nearly identical rays — instead of 64 neighbouring pixels
-// of one sample.  Same camera_dir inputs as k_generate, so the same rays;
-// slot[] maps each position back to the ray's queue slot for its hit.
-__global__ __launch_bounds__(BS) void k_generate_pm(ws_view w, gv4f* __restrict__ out, uint32_t* __restrict__ slot) {
-    const long long total = w.q_off[w.n_jobs];
-    const int stride = gridDim.x * BS;
-    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < total; i += stride) {
-        const int jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
-        const rx_job& J = w.jobs[jx];
-        const int npix = J.camera.width * J.camera.height;
-        const int ns = pass_samples(J);
-        const long long li = i - w.q_off[jx];
-        const int k = (int)(li / ns), s = (int)(li % ns);
-        const long long local = (long long)s * npix + k;
-        const int pix = slot_pixel(J.camera, k);
-        f3 rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, J.first_sample + s);
-        f3 org = ld3(J.camera.pos);
-        RX_GLOBAL gv4f* q = (RX_GLOBAL gv4f*)out + 2 * i;
-        q[0] = gv4f{org.x, org.y, org.z, __int_as_float((int)(w.path_off[jx] + local))};
-        q[1] = gv4f{rd.x, rd.y, rd.z,
-                    __int_as_float(camera_tile(J.camera, pix % J.camera.width, pix / J.camera.width))};
-        slot[i] = (uint32_t)(w.q_off[jx] + local);
-    }
+// of one sample.  The ray is k_generate's for that pixel and sample (same
+// camera_dir inputs).  Returns its path; sets the pixel's tile tag and the
+// (sample-major) queue slot of its hit, where k_shade reads it.
+__device__ __forceinline__ int pm_camera_ray(const ws_view& w, int i, f3& ro, f3& rd, int& tag, int& hit) {
+    const int jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
+    const rx_job& J = w.jobs[jx];
+    const int npix = J.camera.width * J.camera.height;
+    const int ns = pass_samples(J);
+    const int li = i - w.q_off[jx];
+    const int k = li / ns, s = li - k * ns;
+    const int local = s * npix + k;
+    const int pix = slot_pixel(J.camera, k);
+    const int px = pix % J.camera.width, py = pix / J.camera.width;
+    rd = camera_dir(J.camera, px, py, J.first_sample + s);
+    ro = ld3(J.camera.pos);
+    tag = camera_tile(J.camera, px, py);
+    hit = w.q_off[jx] + local;
+    return w.path_off[jx] + local;
 }
 
 // Closest analytic primitive along a ray: the linear scan of
@@ -1051,7 +1048,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     unsigned long long rays = 0;
     lane_rays jr;
     bool exhausted = false;
-    int i = -1;  // queue slot of this lane's ray, -1 when idle
+    int i = -1;  // queue slot of this lane's hit, -1 when idle
     int p = 0;
     const rx_scene* sc = nullptr;
     const rx_node* nodes = nullptr;
@@ -1065,10 +1062,14 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
         int mine = exhausted ? n : wave_fetch(fetch, n, cap, idle, exhausted, seg, tried);
         {
             if (idle && mine < n) {
-                i = mine;
                 int tag = 0;
-                p = EYE ? get_ray_tag(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd, tag)
-                        : get_ray(w, qi, w.ray_slot ? (int)w.ray_slot[i] : i, ro, rd);
+                if (w.cam_gen) {
+                    p = pm_camera_ray(w, mine, ro, rd, tag, i);
+                } else {
+                    const int rs = w.ray_slot ? (int)w.ray_slot[mine] : mine;
+                    p = EYE ? get_ray_tag(w, qi, rs, ro, rd, tag) : get_ray(w, qi, rs, ro, rd);
+                    i = w.hit_slot ? (int)w.hit_slot[mine] : mine;
+                }
                 const int jx = job_of_path(w, p);
                 sc = &w.jobs[jx].scene;
                 rays++;
@@ -1076,7 +1077,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 // primitive scan first (intersect.py:259-287), or its
                 // result from k_prim_scan
                 if (PRE) {
-                    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[w.hit_slot ? (int)w.hit_slot[i] : i];
+                    gv4f h = ((const RX_GLOBAL gv4f*)w.h_rec)[i];
                     prim_t = h.x;
                     prim_id = __float_as_int(h.y);
                 } else if (EYE) {
@@ -1175,7 +1176,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     u = best_u;
                     v = best_v;
                 }
-                put_hit(w, w.hit_slot ? (int)w.hit_slot[i] : i, t, kind, id, u, v);
+                put_hit(w, i, t, kind, id, u, v);
                 if (w.ray_per_path) w.p_rays[p] += 1;
                 i = -1;
             }
@@ -2158,25 +2159,22 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
     RX_STAGE(tm, 0, (k_pass_begin<<<1, PB, 0, s>>>(w)));
     if (w.tl_nodes) RX_STAGE(tm, 0, (k_treelet<<<1, TLB, 0, s>>>(pl.tl_nodes, pl.tl_header, w)));
     // passes of several samples per pixel trace their primary rays
-    // pixel-major (k_generate_pm); later bounces are sorted below
-    const bool pm = pl.sort_rays && w.q_sorted && pl.max_spp > 1;
-    // traced from the pixel-major copy and shaded from the camera: the
-    // sample-major records would only be read by the primitive pre-pass
+    // pixel-major, generated by k_extend itself (pm_camera_ray); later
+    // bounces are sorted below
+    const bool pm = pl.sort_rays && w.si_in && pl.max_spp > 1;
+    // traced pixel-major and shaded from the camera: the sample-major
+    // records would only be read by the primitive pre-pass
     w.cam_rays = pm && !pl.prim_prepass ? 1 : 0;
     // k_generate has nothing to write for a single job whose camera rays
-    // are traced from the pixel-major copy and shaded from the camera, with
-    // rays counted per wave and every path shaded at depth 0
+    // are traced pixel-major and shaded from the camera, with rays counted
+    // per wave and every path shaded at depth 0
     if (!(w.n_jobs == 1 && w.cam_rays && !w.ray_per_path && pl.max_depth >= 1))
         RX_STAGE(tm, 0, (k_generate<<<gp, BS, 0, s>>>(w)));
-    if (pm) RX_STAGE(tm, 6, (k_generate_pm<<<gp, BS, 0, s>>>(w, w.q_sorted, w.si_in)));
     for (int d = 0; d < pl.max_depth; ++d) {
         int qi = d & 1;
         ws_view we = w;
-        if (d == 0 && pm) {
-            we.q_ray[0] = w.q_sorted;
-            we.hit_slot = w.si_in;
-        }
-        if (d > 0 && pl.sort_rays && w.q_sorted && w.P >= SORT_MIN_PATHS) {
+        if (d == 0 && pm) we.cam_gen = 1;
+        if (d > 0 && pl.sort_rays && w.si_in && w.P >= SORT_MIN_PATHS) {
             RX_STAGE(tm, 6, (k_ray_keys<<<gp, BS, 0, s>>>(w, qi, key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
             int rs = tm.begin(6);
             if (rs) return rs;
@@ -2214,7 +2212,7 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
         RX_STAGE(tm, 2, (k_shade<<<grid_for(w.P, SHADE_BS, 16 * BS / SHADE_BS), SHADE_BS, 0, s>>>(w, qi, d)));
         if (d < pl.nee_depth) {
             ws_view wc = w;
-            if (pl.sort_rays && pl.sort_nee && w.q_sorted && w.P >= SORT_MIN_PATHS) {
+            if (pl.sort_rays && pl.sort_nee && w.si_in && w.P >= SORT_MIN_PATHS) {
                 // the shadow rays, grouped by light and hit point (with one
                 // light, shade's order is as coherent and costs no sort)
                 RX_STAGE(tm, 6, (k_nee_keys<<<gp, BS, 0, s>>>(w, d, nee_key_origin_bits(w.n_jobs), w.sk_in, w.si_in)));
