@@ -414,28 +414,37 @@ __device__ __forceinline__ int wave_fetch(int32_t* ctr, int n, int cap, bool idl
     return n;
 }
 
-// Appends to a queue with one atomic per block: every thread of the block
-// calls this at the same point (uniform control flow).  Slots are handed
-// out in thread order within the block.
+// Appends to two queues with one atomic per block and queue: every thread
+// of the block calls this at the same point (uniform control flow), with
+// whether it wants a slot in queue a and in queue b.  Slots are handed out
+// in thread order within the block.  One pair of barriers serves both
+// queues, and the two atomics leave in one instruction (lanes 0 and 1 of
+// wave 0), so their round trips overlap.
 template <int SB = BS>
-__device__ __forceinline__ int block_append(int32_t* counter, bool want, int32_t* lds /* SB / 64 + 1 */) {
-    unsigned long long m = __ballot(want);
+__device__ __forceinline__ void block_append_pair(int32_t* ca, bool wa, int32_t* cb, bool wb,
+                                                  int32_t* lds /* 2 (SB / 64 + 1) */, int& ra, int& rb) {
+    constexpr int NW = SB / 64;
+    unsigned long long ma = __ballot(wa), mb = __ballot(wb);
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) lds[wv] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int32_t tot = 0;
-        for (int k = 0; k < SB / 64; ++k) {
-            int32_t c = lds[k];
-            lds[k] = tot;
-            tot += c;
-        }
-        lds[SB / 64] = tot > 0 ? atomicAdd(counter, tot) : 0;
+    if (lane == 0) {
+        lds[wv] = __popcll(ma);
+        lds[NW + 1 + wv] = __popcll(mb);
     }
     __syncthreads();
-    int r = lds[SB / 64] + lds[wv] + lanes_below(m);
+    if (threadIdx.x < 2) {
+        int32_t* l = lds + threadIdx.x * (NW + 1);
+        int32_t tot = 0;
+        for (int k = 0; k < NW; ++k) {
+            int32_t c = l[k];
+            l[k] = tot;
+            tot += c;
+        }
+        l[NW] = tot > 0 ? atomicAdd(threadIdx.x == 0 ? ca : cb, tot) : 0;
+    }
     __syncthreads();
-    return r;
+    ra = lds[NW] + lds[wv] + lanes_below(ma);
+    rb = lds[2 * NW + 1] + lds[NW + 1 + wv] + lanes_below(mb);
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------- pass control
@@ -1396,7 +1405,7 @@ __device__ __forceinline__ void shade_hit(const ws_view& w, const rx_job& J, int
 __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHADE_WAVES_PER_EU))) void k_shade(
     ws_view w, int qi, int depth) {
     constexpr int BS = SHADE_BS;
-    __shared__ int32_t app[BS / 64 + 1];
+    __shared__ int32_t app[2 * (BS / 64 + 1)];
     const int n = w.counts[qi];
     // the next bounce's extend hands out its queue from the first
     // FETCH_SEGS counters, this bounce's connect from the others
@@ -1442,11 +1451,11 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
             w.col_y[p] = col.y;
             w.col_z[p] = col.z;
         }
-        int slot = block_append<BS>(&w.counts[qo], o.cont, app);
+        int slot, es;
+        block_append_pair<BS>(&w.counts[qo], o.cont, &w.counts[2], o.want_nee, app, slot, es);
         if (o.cont) {
             put_ray(w, qo, slot, p, o.ro2, o.rd2);
         }
-        int es = block_append<BS>(&w.counts[2], o.want_nee, app);
         if (o.want_nee) {
             RX_GLOBAL gv4f* e = (RX_GLOBAL gv4f*)w.e_rec + 4 * (long long)es;
             e[0] = gv4f{o.hit_p.x, o.hit_p.y, o.hit_p.z, __int_as_float(p)};
