@@ -1402,8 +1402,37 @@ __device__ __forceinline__ void shade_hit(const ws_view& w, const rx_job& J, int
     if (depth == 0 && m.type != RX_MAT_DIELECTRIC) w.ior[p] = 1.0f;
 }
 
+// Shading of queue slot i (path p of job J) at bounce `depth`: the ray
+// (a camera ray is recomputed from its pixel and sample), its hit, the
+// path's throughput and colour, then shade_hit; stores the colour.
+__device__ __forceinline__ void shade_slot(const ws_view& w, const rx_job& J, int jx, int i, int p, int depth,
+                                           bool cam, f3 ro, f3 rd, shade_out& o) {
+    int npix = J.camera.width * J.camera.height;
+    int local = p - w.path_off[jx];
+    int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
+    if (cam) {
+        // k_generate's camera ray of this pixel and sample
+        ro = ld3(J.camera.pos);
+        rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, frame);
+    }
+    float t, hu, hv;
+    int kind, id;
+    get_hit(w, i, t, kind, id, hu, hv);
+    // a camera ray's path starts with throughput 1 and colour 0
+    f3 thr = depth == 0 ? mk(1.0f, 1.0f, 1.0f) : mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
+    f3 col = depth == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(w.col_x[p], w.col_y[p], w.col_z[p]);
+    shade_hit(w, J, p, pix, frame, depth, ro, rd, t, kind, id, hu, hv, thr, col, o);
+    w.col_x[p] = col.x;
+    w.col_y[p] = col.y;
+    w.col_z[p] = col.z;
+}
+
+// `jobs` is w.jobs, passed on its own as a read-only, unaliased pointer:
+// when every path of a wave belongs to one job, the job's record (camera,
+// render parameters, scene pointers) is read with scalar loads at a
+// wave-uniform address instead of one vector load per field and lane.
 __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHADE_WAVES_PER_EU))) void k_shade(
-    ws_view w, int qi, int depth) {
+    ws_view w, const rx_job* __restrict__ jobs, int qi, int depth) {
     constexpr int BS = SHADE_BS;
     __shared__ int32_t app[2 * (BS / 64 + 1)];
     const int n = w.counts[qi];
@@ -1416,13 +1445,12 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
     for (int base = xcd_block() * BS; base < n; base += stride) {
         int i = base + threadIdx.x;
         bool valid = i < n;
-        int p = 0;
+        int p = 0, jx = 0;
         shade_out o;
         o.cont = o.want_nee = false;
+        f3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro;
+        const bool cam = depth == 0 && w.cam_rays;
         if (valid) {
-            f3 ro, rd;
-            const bool cam = depth == 0 && w.cam_rays;
-            int jx;
             if (cam) {
                 // primary slot i: path path_off + (i - q_off) of its job
                 jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
@@ -1431,25 +1459,12 @@ __global__ __launch_bounds__(SHADE_BS) __attribute__((amdgpu_waves_per_eu(RX_SHA
                 p = get_ray(w, qi, i, ro, rd);
                 jx = job_of_path(w, p);
             }
-            const rx_job& J = w.jobs[jx];
-            int npix = J.camera.width * J.camera.height;
-            int local = p - w.path_off[jx];
-            int pix = slot_pixel(J.camera, local % npix), frame = J.first_sample + local / npix;
-            if (cam) {
-                // k_generate's camera ray of this pixel and sample
-                ro = ld3(J.camera.pos);
-                rd = camera_dir(J.camera, pix % J.camera.width, pix / J.camera.width, frame);
-            }
-            float t, hu, hv;
-            int kind, id;
-            get_hit(w, i, t, kind, id, hu, hv);
-            // a camera ray's path starts with throughput 1 and colour 0
-            f3 thr = depth == 0 ? mk(1.0f, 1.0f, 1.0f) : mk(w.thr_x[p], w.thr_y[p], w.thr_z[p]);
-            f3 col = depth == 0 ? mk(0.0f, 0.0f, 0.0f) : mk(w.col_x[p], w.col_y[p], w.col_z[p]);
-            shade_hit(w, J, p, pix, frame, depth, ro, rd, t, kind, id, hu, hv, thr, col, o);
-            w.col_x[p] = col.x;
-            w.col_y[p] = col.y;
-            w.col_z[p] = col.z;
+        }
+        const int j0 = __builtin_amdgcn_readfirstlane(jx);
+        if (__ballot(valid && jx != j0) == 0ull) {
+            if (valid) shade_slot(w, jobs[j0], j0, i, p, depth, cam, ro, rd, o);
+        } else if (valid) {
+            shade_slot(w, jobs[jx], jx, i, p, depth, cam, ro, rd, o);
         }
         int slot, es;
         block_append_pair<BS>(&w.counts[qo], o.cont, &w.counts[2], o.want_nee, app, slot, es);
@@ -2218,7 +2233,7 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
             RX_STAGE(tm, 1, RX_EXTEND(false, false));
         }
 #undef RX_EXTEND
-        RX_STAGE(tm, 2, (k_shade<<<grid_for(w.P, SHADE_BS, 16 * BS / SHADE_BS), SHADE_BS, 0, s>>>(w, qi, d)));
+        RX_STAGE(tm, 2, (k_shade<<<grid_for(w.P, SHADE_BS, 16 * BS / SHADE_BS), SHADE_BS, 0, s>>>(w, w.jobs, qi, d)));
         if (d < pl.nee_depth) {
             ws_view wc = w;
             if (pl.sort_rays && pl.sort_nee && w.si_in && w.P >= SORT_MIN_PATHS) {
