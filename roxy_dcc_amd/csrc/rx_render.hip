@@ -612,10 +612,14 @@ __global__ __launch_bounds__(BS) void k_generate(ws_view w) {
 // few pixels — nearly identical rays — instead of 64 neighbouring pixels
 // of one sample.  The ray is k_generate's for that pixel and sample (same
 // camera_dir inputs).  Returns its path; sets the pixel's tile tag and the
-// (sample-major) queue slot of its hit, where k_shade reads it.
-__device__ __forceinline__ int pm_camera_ray(const ws_view& w, int i, f3& ro, f3& rd, int& tag, int& hit) {
-    const int jx = w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i);
-    const rx_job& J = w.jobs[jx];
+// (sample-major) queue slot of its hit, where k_shade reads it.  ONE: the
+// pass holds one job, whose record `jobs` (read-only, unaliased) then
+// gives through scalar loads.
+template <bool ONE>
+__device__ __forceinline__ int pm_camera_ray(const ws_view& w, const rx_job* __restrict__ jobs, int i, f3& ro, f3& rd,
+                                             int& tag, int& hit) {
+    const int jx = ONE ? 0 : (w.n_jobs == 1 ? 0 : owner(w.q_off, w.n_jobs, i));
+    const rx_job& J = jobs[jx];
     const int npix = J.camera.width * J.camera.height;
     const int ns = pass_samples(J);
     const int li = i - w.q_off[jx];
@@ -1032,9 +1036,11 @@ __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* l
 // bvh_closest (rendering/taichi/intersect.py:81-152), so hits are
 // unchanged.
 // EYE: the camera-ray launch of a single-job render, on the eye-relative
-// copies (k_eye_prep).
+// copies (k_eye_prep); its job record (`jobs`, w.jobs passed as a
+// read-only, unaliased pointer) is then read with scalar loads.
 template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE>
-__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(ws_view w, int qi) {
+__global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(
+    ws_view w, const rx_job* __restrict__ jobs, int qi) {
     constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
     __shared__ int2 stack_mem[SL * SBS];
     __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
@@ -1073,14 +1079,14 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
             if (idle && mine < n) {
                 int tag = 0;
                 if (w.cam_gen) {
-                    p = pm_camera_ray(w, mine, ro, rd, tag, i);
+                    p = pm_camera_ray<EYE>(w, jobs, mine, ro, rd, tag, i);
                 } else {
                     const int rs = w.ray_slot ? (int)w.ray_slot[mine] : mine;
                     p = EYE ? get_ray_tag(w, qi, rs, ro, rd, tag) : get_ray(w, qi, rs, ro, rd);
                     i = w.hit_slot ? (int)w.hit_slot[mine] : mine;
                 }
-                const int jx = job_of_path(w, p);
-                sc = &w.jobs[jx].scene;
+                const int jx = EYE ? 0 : job_of_path(w, p);
+                sc = &jobs[jx].scene;
                 rays++;
                 if (!w.ray_per_path) jr.add(w, jx, 1);
                 // primitive scan first (intersect.py:259-287), or its
@@ -2217,8 +2223,8 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
         // eye-relative copies serve them when the workspace holds them
         const bool eye = d == 0 && w.eye_header != nullptr;
 #define RX_EXTEND(C, PR)                                                              \
-    (eye ? (k_extend<EBS, TLN, C, PR, true><<<ge, EBS, 0, s>>>(we, qi))               \
-         : (k_extend<EBS, TLN, C, PR, false><<<ge, EBS, 0, s>>>(we, qi)))
+    (eye ? (k_extend<EBS, TLN, C, PR, true><<<ge, EBS, 0, s>>>(we, we.jobs, qi))     \
+         : (k_extend<EBS, TLN, C, PR, false><<<ge, EBS, 0, s>>>(we, we.jobs, qi)))
         if (pl.prim_prepass) {
             if (pl.count) {
                 RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
