@@ -425,6 +425,26 @@ def test_render_prim_prepass_matches_oracle():
         assert g[k] == c[k], k
 
 
+@pytest.mark.parametrize("width,height", [(96, 64), (384, 216)])
+def test_render_prim_prepass_pixel_major_matches_oracle(width, height):
+    """The pre-pass in passes whose camera rays the traversal computes
+    pixel-major (a BVH of at least 16384 triangles, two samples per pass):
+    each camera ray reads the pre-pass's result at its sample-major hit
+    slot.  At 384x216 the camera rays also run on the eye-relative copies."""
+    from roxy_dcc_amd import _native as nat
+    from roxy_dcc_amd import scenes
+
+    world_tuple = scenes.large_scene(seed=6, n_meshes=40, seg_u=20, seg_v=11, n_spheres=16)
+    print(f"large/prepass pixel-major at {width}x{height}", flush=True)
+    res = helpers.render_pair("large", width=width, height=height, samples=4, spp_per_pass=2,
+                              world_tuple=world_tuple, flags=nat.RX_FLAG_PRIM_PREPASS | nat.RX_FLAG_COUNT_TRAFFIC)
+    np.testing.assert_array_equal(res["gpu_color"], res["cpu_color"])
+    np.testing.assert_array_equal(res["gpu_depth"], res["cpu_depth"])
+    g, c = res["gpu_counts"], res["cpu_counts"]
+    for k in ("rays", "node_pops", "tri_tests", "prim_tests", "shadow_rays", "shadow_prim_tests"):
+        assert g[k] == c[k], k
+
+
 @pytest.mark.parametrize("radius,dilation,guides", [(1, 1, "none"), (1, 1, "all"), (2, 1, "normal"),
                                                     (1, 2, "all"), (1, 4, "albedo"), (1, 16, "depth"),
                                                     (3, 3, "all")])
