@@ -2140,11 +2140,15 @@ inline unsigned nee_key_bits(int n_jobs) {
 }
 
 // Queues shorter than this are traced in shade's order (the sort kernels
-// see the length on the device and leave the identity permutation), and
-// passes of fewer path slots than SORT_MIN_PATHS are not reordered at all:
-// for them the sort's launches cost more than they save (a loop of
-// one-sample passes of small frames would pay them thousands of times).
-constexpr int SORT_MIN_RAYS = 8192;
+// see the length on the device and leave the identity permutation): a
+// short queue is spread a few rays per wave over the whole grid, where
+// the sort's four passes cost more than its order saves (2M-triangle
+// scene, ms per render: 21.70-21.97 with 8192, 21.50 with 2^17, 21.42
+// with 2^18 and with 2^19; the mesh within noise).  Passes of fewer path
+// slots than SORT_MIN_PATHS are not reordered at all: for them the sort's
+// launches cost more than they save (a loop of one-sample passes of small
+// frames would pay them thousands of times).
+constexpr int SORT_MIN_RAYS = 1 << 18;
 constexpr long long SORT_MIN_PATHS = 1 << 20;
 // Nor are passes whose largest BVH has fewer triangles than this: its
 // nodes and triangles stay in the L1 and L2 caches whatever the ray order
