@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define RX_ABI_VERSION 5
+#define RX_ABI_VERSION 6
 
 /* status codes */
 #define RX_OK 0
@@ -163,7 +163,13 @@ typedef struct rx_texture {
  * be 16-byte aligned (records are read as 16-byte words); the entry points
  * return RX_ERR_INVALID otherwise.  tri_uv (6 floats per triangle in leaf
  * order: uv0, uv1, uv2), textures and texels are needed only when a
- * material has a texture (material.texture >= 0, trace.py:129-136). */
+ * material has a texture (material.texture >= 0, trace.py:129-136).
+ * flags: RX_SCENE_FINITE_BOXES when the caller knows every triangle vertex
+ * (so every BVH box coordinate) is finite; rx_render may then trace the
+ * tree from a copy without child links (see rx_render_workspace_size_for_
+ * scene).  Set it only when it holds: a NaN box coordinate would be read as
+ * a leaf mark in that copy.  0 is always safe. */
+#define RX_SCENE_FINITE_BOXES 1
 typedef struct rx_scene {
     const rx_node* nodes;
     const rx_bvh_header* header;
@@ -183,7 +189,7 @@ typedef struct rx_scene {
     int32_t n_lights;
     int32_t bvh_depth;
     int32_t n_textures;
-    int32_t pad;
+    int32_t flags;
 } rx_scene;
 
 /* Pinhole camera (rendering/taichi/camera.py:23-30). */
@@ -376,7 +382,15 @@ int rx_render_workspace_size(int width, int height, int samples_per_pass,
  * the intersection tests built from them alone (64 bytes per node, 48 per
  * triangle, 32 per analytic primitive), and with 1..256 analytic
  * primitives 32 bytes per 8x8 pixel tile: the primitives the tile's camera
- * rays can hit, the only ones they then test.  Same images, ray counts,
+ * rays can hit, the only ones they then test.  When the scene has
+ * RX_SCENE_FINITE_BOXES, a tree of depth d >= 1 and at most 4 * n_nodes
+ * heap slots (2^(d+1) - 1), the room holds instead of the node copy a
+ * link-free copy of the tree that every traversal of the call reads: the
+ * interior nodes' child boxes in heap order (root slot 0, children of slot
+ * h at 2h+1 and 2h+2; 64 bytes per interior slot, 2^d - 1 of them, in
+ * world space and again minus the camera position), a leaf child's box
+ * marked on its x axis in a way the slab test does not see, and 4 bytes
+ * per heap slot for the leaves' triangle ranges.  Same images, ray counts,
  * node pops and triangle tests either way (primitive tests can only drop);
  * a workspace of rx_render_workspace_size() bytes renders without them.
  * No counterpart in the reference (render_kernel recomputes these per ray

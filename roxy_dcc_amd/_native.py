@@ -34,6 +34,7 @@ RX_FLAG_SMALL_BLOCKS = 8
 RX_FLAG_ADAPTIVE = 16
 RX_FLAG_KEEP_QUEUE_ORDER = 32
 RX_NOALLOC_SCRATCH_BYTES = 768 << 20
+RX_SCENE_FINITE_BOXES = 1
 
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -52,7 +53,7 @@ class RxScene(C.Structure):
         ("nodes", vp), ("header", vp), ("tris", vp), ("tri_shade", vp), ("materials", vp),
         ("prims", vp), ("prim_materials", vp), ("lights", vp), ("tri_uv", vp), ("textures", vp), ("texels", vp),
         ("n_nodes", i32), ("n_tris", i32), ("n_materials", i32), ("n_prims", i32), ("n_lights", i32),
-        ("bvh_depth", i32), ("n_textures", i32), ("pad", i32),
+        ("bvh_depth", i32), ("n_textures", i32), ("flags", i32),
     ]
 
 

@@ -150,6 +150,18 @@ struct ws_view {
     // (k_prim_tiles); null otherwise
     uint64_t* tile_prims;
     int tile_cols;
+    // link-free copy of the (single) job's BVH (imp_usable; null when not
+    // used): slot h of heap order (root 0, children 2h+1 and 2h+2) holds,
+    // for an interior node, its children's boxes as the node record's
+    // first three 16-byte words (16 bytes apart from the next record's),
+    // a leaf child's box marked on its x axis (imp_mark); imp_nodes in
+    // world space, eye_imp minus the camera position (with the eye-relative
+    // copies); for a leaf, its packed (first << 8 | count) in imp_leaf.
+    // imp_records = 2^depth - 1 interior slots.
+    gv4f* imp_nodes;
+    gv4f* eye_imp;
+    int32_t* imp_leaf;
+    int imp_records;
 };
 
 void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_view& w, bool sort = true) {
@@ -165,6 +177,9 @@ void carve(rx_arena& ar, int n_jobs, long long P, bool deep, int err_stride, ws_
     w.eye_prims = nullptr;
     w.tile_prims = nullptr;
     w.tile_cols = 0;
+    w.imp_nodes = w.eye_imp = nullptr;
+    w.imp_leaf = nullptr;
+    w.imp_records = 0;
     w.path_off = ar.take<int32_t>(n_jobs + 1);
     w.pix_off = ar.take<int32_t>(n_jobs + 1);
     w.counts = ar.take<int32_t>(8);
@@ -752,7 +767,7 @@ __global__ __launch_bounds__(BS) void k_eye_prep(ws_view w) {
     const rx_job& J = w.jobs[0];
     const rx_scene& sc = J.scene;
     const f3 e = ld3(J.camera.pos);
-    const int nn = sc.n_tris > 0 ? sc.n_nodes : 0;
+    const int nn = sc.n_tris > 0 && w.eye_nodes ? sc.n_nodes : 0;
     long long n = nn > sc.n_tris ? nn : sc.n_tris;
     if (sc.n_prims > n) n = sc.n_prims;
     if (n < 1) n = 1;
@@ -790,6 +805,89 @@ __global__ __launch_bounds__(BS) void k_eye_prep(ws_view w) {
             }
             *w.eye_header = h;
         }
+    }
+}
+
+// The link-free tree copy (imp_nodes, eye_imp, imp_leaf).  A leaf child's
+// box (lo, hi) is marked on its x axis: lo.x and hi.x swapped when they
+// differ, lo.x a quiet NaN when they are equal.  The slab tests take
+// fminf / fmaxf of the distances to the two x planes, the same pair either
+// way round; with one of them NaN, fminf and fmaxf return the other, the
+// distance to both planes when they are equal.  So a marked box gives
+// every slab test the bits the box itself gives.  A box of finite
+// coordinates (RX_SCENE_FINITE_BOXES) has lo.x <= hi.x, so !(lo.x <= hi.x)
+// reads the mark, and no child link is stored.
+constexpr uint32_t IMP_LEAF_NAN = 0x7fc0f1a9u;
+constexpr int IMP_MAX_DEPTH = 24;
+
+__device__ __forceinline__ void imp_mark(float& lo, float& hi) {
+    if (lo < hi) {
+        const float t = lo;
+        lo = hi;
+        hi = t;
+    } else {
+        lo = __uint_as_float(IMP_LEAF_NAN);
+    }
+}
+
+// Links of interior slot h's children (~slot for a leaf), from the marks of
+// its record's child boxes.
+__device__ __forceinline__ gv2i imp_links(gv4f a, gv4f b, gv4f c, int h) {
+    const int l = 2 * h + 1, r = l + 1;
+    return gv2i{a.x <= a.w ? l : ~l, b.z <= c.y ? r : ~r};
+}
+
+// Writes the three record words of a node's child boxes (left lo, hi;
+// right lo, hi) at slot h, marking the leaf children.
+__device__ __forceinline__ void imp_put(RX_GLOBAL gv4f* dst, long long h, float b[12], bool lleaf, bool rleaf) {
+    if (lleaf) imp_mark(b[0], b[3]);
+    if (rleaf) imp_mark(b[6], b[9]);
+    dst[4 * h] = gv4f{b[0], b[1], b[2], b[3]};
+    dst[4 * h + 1] = gv4f{b[4], b[5], b[6], b[7]};
+    dst[4 * h + 2] = gv4f{b[8], b[9], b[10], b[11]};
+}
+
+// The link-free copy of job 0's BVH: one thread per heap slot h walks from
+// the root along the bits of h + 1 below its leading one (0: left child,
+// 1: right) through the node records' links; a slot below a leaf stays
+// unused.  An interior node's slot gets its record's child boxes (and,
+// with the eye-relative copies, those boxes minus the camera position,
+// each marked after the subtraction), a leaf's slot its packed triangle
+// range.
+__global__ __launch_bounds__(BS) void k_imp_build(ws_view w) {
+    const rx_job& J = w.jobs[0];
+    const rx_scene& sc = J.scene;
+    const f3 e = ld3(J.camera.pos);
+    const long long slots = 2LL * w.imp_records + 1;
+    for (long long h = (long long)blockIdx.x * BS + threadIdx.x; h < slots; h += (long long)gridDim.x * BS) {
+        const unsigned hp = (unsigned)(h + 1);
+        const int depth = 31 - __clz((int)hp);
+        int link = 0;
+        bool used = true;
+        for (int k = depth - 1; k >= 0; --k) {
+            if (link < 0) {
+                used = false;
+                break;
+            }
+            link = sc.nodes[link].link[(hp >> k) & 1u];
+        }
+        if (!used) continue;
+        if (link < 0) {
+            w.imp_leaf[h] = ~link;
+            continue;
+        }
+        if (h >= w.imp_records) continue;  // deeper than the header's depth: never read
+        const RX_GLOBAL gv4f* src = (const RX_GLOBAL gv4f*)(sc.nodes + link);
+        const gv4f p0 = src[0], p1 = src[1], p2 = src[2];
+        const gv2i ln = ((const RX_GLOBAL gv2i*)src)[6];
+        float b[12] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
+        if (w.eye_imp) {
+            const float ec[3] = {e.x, e.y, e.z};
+            float r[12];
+            for (int q = 0; q < 12; ++q) r[q] = b[q] - ec[q % 3];
+            imp_put((RX_GLOBAL gv4f*)w.eye_imp, h, r, ln.x < 0, ln.y < 0);
+        }
+        imp_put((RX_GLOBAL gv4f*)w.imp_nodes, h, b, ln.x < 0, ln.y < 0);
     }
 }
 
@@ -1007,10 +1105,20 @@ __global__ __launch_bounds__(TLB) void k_treelet(const rx_node* __restrict__ nod
 // count, 0 when the pass has no treelet.
 // EYE: the boxes minus the eye of job 0 (the camera-ray traversal's
 // eye-relative form, as the node copy k_eye_prep writes).
-template <int SBS, int TLN, bool EYE = false>
+// IMP: the first TLN slots of the link-free copy (heap order: the top of
+// the tree), boxes only; slot k of the copy is slot k in LDS.
+template <int SBS, int TLN, bool EYE = false, bool IMP = false>
 __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* link) {
     static_assert(TLN == 0 || TLN == TL_NODES, "the treelet's links address TL_NODES slots");
-    if (TLN == 0 || w.tl_nodes == nullptr) return 0;
+    if (TLN == 0) return 0;
+    if (IMP) {
+        const int n = w.imp_records < TLN ? w.imp_records : TLN;
+        const RX_GLOBAL gv4f* src = (const RX_GLOBAL gv4f*)(EYE ? w.eye_imp : w.imp_nodes);
+        for (int k = threadIdx.x; k < 3 * n; k += SBS) box[k] = src[4 * (k / 3) + k % 3];
+        __syncthreads();
+        return n;
+    }
+    if (w.tl_nodes == nullptr) return 0;
     const int n = w.tl_count[0];
     const f3 e = EYE ? ld3(w.jobs[0].camera.pos) : mk(0.0f, 0.0f, 0.0f);
     for (int k = threadIdx.x; k < 3 * n; k += SBS) {
@@ -1038,7 +1146,11 @@ __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* l
 // EYE: the camera-ray launch of a single-job render, on the eye-relative
 // copies (k_eye_prep); its job record (`jobs`, w.jobs passed as a
 // read-only, unaliased pointer) is then read with scalar loads.
-template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE>
+// IMP: the pass traces the link-free copy of its (single) job's tree
+// (imp_nodes, or eye_imp for the camera rays): node links are heap slots,
+// a leaf's triangle range comes from imp_leaf; the same boxes, tests and
+// visit order.
+template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE, bool IMP>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(
     ws_view w, const rx_job* __restrict__ jobs, int qi) {
     constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
@@ -1046,9 +1158,10 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
     __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
     __shared__ gv2i tl_link_mem[TLN > 0 ? TLN : 1];
     lane_stack<SBS, SL> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x, (int)gridDim.x * SBS};
-    const int tl_n = load_treelet<SBS, TLN, EYE>(w, tl_box_mem, tl_link_mem);
+    const int tl_n = load_treelet<SBS, TLN, EYE, IMP>(w, tl_box_mem, tl_link_mem);
     const RX_LDS gv4f* tl_box = (const RX_LDS gv4f*)tl_box_mem;
     const RX_LDS gv2i* tl_link = (const RX_LDS gv2i*)tl_link_mem;
+    const RX_GLOBAL gv4f* imp = (const RX_GLOBAL gv4f*)(EYE ? w.eye_imp : w.imp_nodes);
     const int n = w.counts[qi];
     int32_t* fetch = w.fetch_seg;
     int seg = home_segment(SBS), tried = 0;
@@ -1117,7 +1230,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                                                  hdr.bmax[1], hdr.bmax[2], best_t, &rt);
                     if (root) {
                         have = true;
-                        cur = TLN > 0 && tl_n > 0 && sc->nodes == w.tl_nodes ? TL_REF : hdr.root_link;
+                        cur = IMP ? 0 : (TLN > 0 && tl_n > 0 && sc->nodes == w.tl_nodes ? TL_REF : hdr.root_link);
                         sp = 0;
                     }
                 }
@@ -1135,7 +1248,19 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                         if (COUNT) cnt.pops++;
                         gv4f a, b, c;
                         gv2i dl;
-                        if (TLN > 0 && cur >= TL_REF) {
+                        if (IMP) {
+                            if (TLN > 0 && cur < tl_n) {
+                                a = tl_box[3 * cur];
+                                b = tl_box[3 * cur + 1];
+                                c = tl_box[3 * cur + 2];
+                            } else {
+                                const RX_GLOBAL gv4f* q = imp + 4 * (long long)cur;
+                                a = q[0];
+                                b = q[1];
+                                c = q[2];
+                            }
+                            dl = imp_links(a, b, c, cur);
+                        } else if (TLN > 0 && cur >= TL_REF) {
                             const int k = cur - TL_REF;
                             a = tl_box[3 * k];
                             b = tl_box[3 * k + 1];
@@ -1155,7 +1280,7 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 }
                 if (have && cur < 0) {
                     if (COUNT) cnt.pops++;
-                    int lw = ~cur;
+                    int lw = IMP ? w.imp_leaf[~cur] : ~cur;
                     int start = lw >> 8, end = start + (lw & 255);
                     for (int t2 = start; t2 < end; ++t2) {
                         float u, v, t;
@@ -2226,9 +2351,13 @@ int enqueue_pass(const ws_view& w0, const pass_plan& pl, rx_counters* ctr, stage
         // the camera rays of a single-job pass start at its eye: its
         // eye-relative copies serve them when the workspace holds them
         const bool eye = d == 0 && w.eye_header != nullptr;
-#define RX_EXTEND(C, PR)                                                              \
-    (eye ? (k_extend<EBS, TLN, C, PR, true><<<ge, EBS, 0, s>>>(we, we.jobs, qi))     \
-         : (k_extend<EBS, TLN, C, PR, false><<<ge, EBS, 0, s>>>(we, we.jobs, qi)))
+        // the link-free copy of the pass's (single) tree, when built
+        const bool imp = w.imp_nodes != nullptr;
+#define RX_EXTEND(C, PR)                                                                         \
+    (imp ? (eye ? (k_extend<EBS, TLN, C, PR, true, true><<<ge, EBS, 0, s>>>(we, we.jobs, qi))        \
+                : (k_extend<EBS, TLN, C, PR, false, true><<<ge, EBS, 0, s>>>(we, we.jobs, qi)))      \
+         : (eye ? (k_extend<EBS, TLN, C, PR, true, false><<<ge, EBS, 0, s>>>(we, we.jobs, qi))       \
+                : (k_extend<EBS, TLN, C, PR, false, false><<<ge, EBS, 0, s>>>(we, we.jobs, qi))))
         if (pl.prim_prepass) {
             if (pl.count) {
                 RX_STAGE(tm, 5, (k_prim_scan<true><<<gp, BS, 0, s>>>(w, qi)));
@@ -2386,14 +2515,28 @@ int carve_into(void* d_workspace, size_t bytes, int n_jobs, long long P, long lo
     return RX_OK;
 }
 
+// Whether a render traces the link-free copy of the scene's tree: its
+// boxes are finite (the scene says so), its root is interior, and the
+// heap slots of its depth are at most four per node (a tree that is not
+// far from balanced, as the Morton and median builds give).
+bool imp_usable(const rx_scene& sc) {
+    return sc.n_tris > 0 && (sc.flags & RX_SCENE_FINITE_BOXES) != 0 && sc.bvh_depth >= 1 &&
+           sc.bvh_depth <= IMP_MAX_DEPTH && (2LL << sc.bvh_depth) - 1 <= 4LL * sc.n_nodes;
+}
+
 // The eye-relative copies of a scene (k_eye_prep), after a render's own
 // workspace: the root record, the nodes, the triangles, 32 bytes per
 // primitive.
 // ... and, for 1..PRIM_TILE_MAX primitives, the per-tile primitive masks
 // of a width x height camera.
+// ... and, when imp_usable, the link-free copy of the tree in world space
+// and minus the eye (64 bytes per interior heap slot each) and 4 bytes
+// per heap slot for the leaves (the eye-relative node records are then
+// not needed).
 void take_eye(rx_arena& ar, const rx_scene& sc, int width, int height, ws_view& w) {
+    const bool imp = imp_usable(sc);
     w.eye_header = ar.take<rx_bvh_header>(1);
-    w.eye_nodes = ar.take<rx_node>(sc.n_tris > 0 ? (size_t)sc.n_nodes : 0);
+    w.eye_nodes = sc.n_tris > 0 && !imp ? ar.take<rx_node>((size_t)sc.n_nodes) : nullptr;
     w.eye_tris = ar.take<rx_tri>((size_t)sc.n_tris);
     w.eye_prims = ar.take<gv4f>(2 * (size_t)sc.n_prims);
     w.tile_prims = nullptr;
@@ -2401,6 +2544,15 @@ void take_eye(rx_arena& ar, const rx_scene& sc, int width, int height, ws_view& 
     if (sc.n_prims >= 1 && sc.n_prims <= PRIM_TILE_MAX) {
         const size_t tiles = (size_t)w.tile_cols * (size_t)((height + 7) >> 3);
         w.tile_prims = ar.take<uint64_t>(PRIM_TILE_WORDS * tiles);
+    }
+    w.imp_nodes = w.eye_imp = nullptr;
+    w.imp_leaf = nullptr;
+    w.imp_records = 0;
+    if (imp) {
+        w.imp_records = (1 << sc.bvh_depth) - 1;
+        w.imp_nodes = ar.take<gv4f>(4 * (size_t)w.imp_records);
+        w.eye_imp = ar.take<gv4f>(4 * (size_t)w.imp_records);
+        w.imp_leaf = ar.take<int32_t>(2 * (size_t)w.imp_records + 1);
     }
 }
 
@@ -2530,6 +2682,10 @@ extern "C" int rx_render(const rx_scene* scene, const rx_camera* camera, const r
         if (j.scene.n_prims > recs) recs = j.scene.n_prims;
         k_eye_prep<<<grid_for(recs > 0 ? recs : 1, BS, 16), BS, 0, s>>>(w);
         RX_CHECK_LAUNCH();
+        if (w.imp_nodes) {
+            k_imp_build<<<grid_for(2LL * w.imp_records + 1, BS, 16), BS, 0, s>>>(w);
+            RX_CHECK_LAUNCH();
+        }
         if (w.tile_prims) {
             const long long tiles = (long long)w.tile_cols * ((camera->height + 7) >> 3);
             k_prim_tiles<<<grid_for(tiles, BS, 16), BS, 0, s>>>(w);

@@ -164,6 +164,9 @@ class DeviceScene:
             self.builder.MAX_LEAF_SIZE = int(max_leaf)
         self.builder.build(flat.batches, method=method)
         self.builder.upload()
+        # finite vertices make finite boxes: rx_render may then trace a copy
+        # of the tree without child links (RX_SCENE_FINITE_BOXES)
+        self.finite = all(bool(np.isfinite(getattr(b, k)).all()) for b in flat.batches for k in ("v0", "v1", "v2"))
 
         def dev(arr):
             if len(arr) == 0:
@@ -220,7 +223,7 @@ class DeviceScene:
                 self.d_textures.data_ptr() if self.d_textures is not None else None,
                 self.d_texels.data_ptr() if self.d_texels is not None else None,
                 b.node_count, b.triangle_count, len(b.mat_palette), self.n_prims, self.n_lights,
-                b.depth, self.n_textures, 0)
+                b.depth, self.n_textures, nat.RX_SCENE_FINITE_BOXES if self.finite else 0)
         return self._struct
 
     def stats(self):

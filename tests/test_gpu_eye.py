@@ -6,7 +6,10 @@ triangle tests, on scenes with triangle meshes, spheres, planes and cubes,
 in both next-event modes.  With the copies, camera rays test only the
 analytic primitives their pixel's 8x8 tile can see, so primitive tests can
 only drop; a scene whose spheres and cubes all lie outside the view has
-no primitive test at all for its camera rays."""
+no primitive test at all for its camera rays.  Scenes marked
+RX_SCENE_FINITE_BOXES trace a link-free copy of the tree (heap order,
+leaf children marked in their boxes): every bit and counter again equals
+the link layout's."""
 from __future__ import annotations
 
 import numpy as np
@@ -18,7 +21,7 @@ from tests import helpers
 pytestmark = pytest.mark.gpu
 
 
-def _render(world, W, H, settings, samples, spp, eye, seed=11, mode=None):
+def _render(world, W, H, settings, samples, spp, eye, seed=11, mode=None, finite=True):
     import torch
 
     from roxy_dcc_amd import renderer as R
@@ -26,6 +29,8 @@ def _render(world, W, H, settings, samples, spp, eye, seed=11, mode=None):
 
     ex = extract_scene(world)
     sc = ex.scene.struct()
+    # the scene's own flag, or 0: the link layout whatever the boxes
+    sc.flags = sc.flags if finite else 0
     records = (sc.n_nodes + sc.n_tris if sc.n_tris > 0 else 0) + sc.n_prims
     # rx_render takes the copies when a call's camera rays are at least four per record
     assert W * H * min(spp, samples) >= 4 * records, (W, H, spp, records)
@@ -42,12 +47,12 @@ def _render(world, W, H, settings, samples, spp, eye, seed=11, mode=None):
         done += n
     torch.cuda.synchronize()
     out = {k: getattr(frame, k).cpu().numpy() for k in ("color", "normal", "albedo", "depth")}
-    return out, integ.counts(), integ.workspace.numel()
+    return out, integ.counts(), integ.workspace.numel(), ex
 
 
 def _same(world, W, H, settings, samples, spp, **kw):
-    a, ca, wa = _render(world, W, H, settings, samples, spp, True, **kw)
-    b, cb, wb = _render(world, W, H, settings, samples, spp, False, **kw)
+    a, ca, wa, _ = _render(world, W, H, settings, samples, spp, True, **kw)
+    b, cb, wb, _ = _render(world, W, H, settings, samples, spp, False, **kw)
     assert wa > wb, "the scene-sized workspace holds the eye-relative copies"
     for k in a:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
@@ -122,3 +127,52 @@ def test_tile_masks_skip_primitives_out_of_view():
     ca, cb = _same(w, 64, 64, st, 2, 2)
     assert 0 < ca["prim_tests"] < cam_rays, ca
     assert cb["prim_tests"] == cam_rays * 9, (ca, cb)
+
+
+def _leaf_boxes_flat_in_x(ex):
+    """Leaf boxes of the scene's tree whose x extent is zero (their mark in
+    the link-free copy is the NaN form)."""
+    from roxy_dcc_amd import _native as nat
+
+    b = ex.scene.builder
+    nodes = b.d["nodes"].cpu().numpy().view(np.int32).reshape(-1, nat.NODE_BYTES // 4)
+    boxes = nodes[:, :12].view(np.float32)
+    links = nodes[:, 12:14]
+    flat = 0
+    for c in (0, 1):
+        leaf = links[:, c] < 0
+        lo, hi = boxes[:, 6 * c], boxes[:, 6 * c + 3]
+        flat += int(np.sum(leaf & (lo == hi)))
+    return flat
+
+
+@pytest.mark.parametrize("name", ["cornell", "mesh", "large"])
+def test_link_free_tree_copy_matches_link_layout(name):
+    """The same render over the link-free copy (scene flag set) and over the
+    node records' links (flag cleared): every buffer bit, the ray count,
+    node pops and triangle tests equal, camera rays on the eye-relative
+    copies and bounces alike.
+    The Cornell box's walls give leaf boxes of zero x extent, whose mark is
+    the NaN form; the mesh's 70k-triangle tree has leaves at two depths."""
+    from roxy_dcc_amd import scenes
+
+    if name == "large":
+        world, W, H, st = scenes.large_scene(seed=30, n_meshes=40, seg_u=20, seg_v=11, n_spheres=16)
+        W, H, samples, spp = 384, 216, 3, 3
+    else:
+        world, W, H, st = helpers.load(name)
+        W, H, samples, spp = (96, 72, 4, 2) if name == "cornell" else (512, 512, 4, 4)
+    print(f"link-free tree copy: {name} at {W}x{H}x{samples}", flush=True)
+    a, ca, _, ex = _render(world, W, H, st, samples, spp, True, finite=True)
+    b, cb, _, _ = _render(world, W, H, st, samples, spp, True, finite=False)
+    assert ex.scene.struct().flags == nat.RX_SCENE_FINITE_BOXES
+    if name == "cornell":
+        assert _leaf_boxes_flat_in_x(ex) > 0
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    # primitive tests under the tile masks depend on which rays a wave takes
+    # together, which the hand-out atomics' timing decides
+    for k in ca:
+        if k != "prim_tests":
+            assert ca[k] == cb[k], (k, ca, cb)
+    assert ca["node_pops"] > 0

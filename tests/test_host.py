@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     lib = nat.lib()
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.rx_version() == 5
+    assert lib.rx_version() == 6
 
 
 def test_scene_workspace_holds_the_eye_relative_copies():
@@ -50,6 +50,33 @@ def test_scene_workspace_holds_the_eye_relative_copies():
         payload = 64 * n_nodes + 48 * n_tris + 32 * n_prims + 32 + 32 * tiles
         assert payload <= extra <= payload + 6 * 256, (extra, payload)
     assert lib.rx_render_workspace_size_for_scene(None, 256, 128, 4, C.byref(a)) == nat.RX_ERR_INVALID
+
+
+def test_scene_workspace_holds_the_link_free_tree_copy():
+    """With RX_SCENE_FINITE_BOXES and a tree whose heap slots (2^(depth+1) - 1)
+    are at most four per node, the scene part holds the link-free copy of
+    the tree instead of the eye-relative node records: 64 bytes per interior
+    heap slot (2^depth - 1) in world space and again minus the eye, 4 bytes
+    per heap slot for the leaves.  Without the flag, or for a tree too far
+    from balanced, the scene part is as before."""
+    lib = nat.lib()
+    for n_nodes, n_tris, n_prims, depth, usable in ((524287, 2_000_000, 67, 18, True), (1000, 2000, 5, 10, True),
+                                                    (1000, 2000, 5, 12, False), (3, 2, 0, 1, True)):
+        for flags in (0, nat.RX_SCENE_FINITE_BOXES):
+            sc = nat.RxScene()
+            sc.n_nodes, sc.n_tris, sc.n_prims, sc.bvh_depth, sc.flags = n_nodes, n_tris, n_prims, depth, flags
+            a, b = C.c_size_t(), C.c_size_t()
+            assert lib.rx_render_workspace_size_for_scene(C.byref(sc), 250, 128, 4, C.byref(a)) == 0
+            assert lib.rx_render_workspace_size(250, 128, 4, depth, C.byref(b)) == 0
+            extra = a.value - b.value
+            tiles = 32 * 16 if 1 <= n_prims <= 256 else 0
+            base = 48 * n_tris + 32 * n_prims + 32 + 32 * tiles
+            if flags and usable:
+                interior = (1 << depth) - 1
+                payload = base + 2 * 64 * interior + 4 * (2 * interior + 1)
+            else:
+                payload = base + 64 * n_nodes
+            assert payload <= extra <= payload + 8 * 256, (n_nodes, depth, flags, extra, payload)
 
 
 def test_morton_shape_closed_form_matches_recursion():
