@@ -72,7 +72,9 @@ inline int trav_grid(long long n, bool deep, int per_cu = 0) {
 // CU at 6), with STACK_LDS_TL stack entries per lane in LDS:
 // 1024 x 6 x 8 B + 511 x 56 B = 76.6 KB and 512 x 6 x 8 B + 28.6 KB =
 // 52.6 KB, within a CU's 160 KB at those block counts (TL_NODES and
-// TL_REF are in rx_device.h).
+// TL_REF are in rx_device.h).  Over the link-free tree copy the
+// closest-hit treelet keeps boxes only (48 B per slot), and the lanes get
+// a seventh LDS stack entry: 1024 x 7 x 8 B + 511 x 48 B = 80.0 KB.
 #ifndef RX_STACK_LDS_TL
 #define RX_STACK_LDS_TL 6
 #endif

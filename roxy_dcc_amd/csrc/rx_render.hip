@@ -1153,10 +1153,12 @@ __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* l
 template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE, bool IMP>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(
     ws_view w, const rx_job* __restrict__ jobs, int qi) {
-    constexpr int SL = TLN > 0 ? STACK_LDS_TL : STACK_LDS;
+    // the link-free copy's treelet has no links: its 8 bytes per slot and
+    // the 8 fewer per box record give the stack one more LDS entry per lane
+    constexpr int SL = TLN > 0 ? STACK_LDS_TL + (IMP ? 1 : 0) : STACK_LDS;
     __shared__ int2 stack_mem[SL * SBS];
     __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
-    __shared__ gv2i tl_link_mem[TLN > 0 ? TLN : 1];
+    __shared__ gv2i tl_link_mem[TLN > 0 && !IMP ? TLN : 1];
     lane_stack<SBS, SL> stk{stack_mem + threadIdx.x, w.spill + blockIdx.x * SBS + threadIdx.x, (int)gridDim.x * SBS};
     const int tl_n = load_treelet<SBS, TLN, EYE, IMP>(w, tl_box_mem, tl_link_mem);
     const RX_LDS gv4f* tl_box = (const RX_LDS gv4f*)tl_box_mem;
