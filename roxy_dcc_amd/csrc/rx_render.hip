@@ -1154,8 +1154,8 @@ __device__ __forceinline__ int load_treelet(const ws_view& w, gv4f* box, gv2i* l
 template <int SBS, int TLN, bool COUNT, bool PRE, bool EYE, bool IMP>
 __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVES_PER_EU))) void k_extend(
     ws_view w, const rx_job* __restrict__ jobs, int qi) {
-    // the link-free copy's treelet has no links: its 8 bytes per slot and
-    // the 8 fewer per box record give the stack one more LDS entry per lane
+    // the link-free copy's treelet keeps boxes only (48 of 56 bytes per
+    // slot), which leaves room for one more LDS stack entry per lane
     constexpr int SL = TLN > 0 ? STACK_LDS_TL + (IMP ? 1 : 0) : STACK_LDS;
     __shared__ int2 stack_mem[SL * SBS];
     __shared__ gv4f tl_box_mem[TLN > 0 ? 3 * TLN : 1];
