@@ -1206,7 +1206,9 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                     p = EYE ? get_ray_tag(w, qi, rs, ro, rd, tag) : get_ray(w, qi, rs, ro, rd);
                     i = w.hit_slot ? (int)w.hit_slot[mine] : mine;
                 }
-                const int jx = EYE ? 0 : job_of_path(w, p);
+                // camera-ray launches and passes over the link-free copy
+                // are single-job renders: job 0, its record in scalar loads
+                const int jx = EYE || IMP ? 0 : job_of_path(w, p);
                 sc = &jobs[jx].scene;
                 rays++;
                 if (!w.ray_per_path) jr.add(w, jx, 1);
@@ -1229,7 +1231,8 @@ __global__ __launch_bounds__(SBS) __attribute__((amdgpu_waves_per_eu(EXTEND_WAVE
                 if (sc->n_tris > 0) {
                     nodes = EYE ? w.eye_nodes : sc->nodes;
                     tris = EYE ? w.eye_tris : sc->tris;
-                    const rx_bvh_header hdr = gload(EYE ? w.eye_header : sc->header);
+                    const rx_bvh_header hdr = EYE || IMP ? cload(EYE ? w.eye_header : sc->header)
+                                                         : gload(sc->header);
                     inv = inv_dir(rd);
                     float rt;
                     const bool root = EYE ? slab_rel(inv, hdr.bmin[0], hdr.bmin[1], hdr.bmin[2], hdr.bmax[0],
