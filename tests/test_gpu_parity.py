@@ -367,6 +367,41 @@ def test_untiled_frame_pixel_major_matches_oracle(width, height, spp):
     assert res["gpu_counts"]["shadow_rays"] == res["cpu_counts"]["shadow_rays"]
 
 
+def test_headline_render_matches_oracle_on_sampled_pixels():
+    """The benchmarked workload itself, the 2M-triangle scene at 1920x1080
+    and 16 spp in one pass (the link-free tree copy, eye-relative camera
+    rays, tile masks, sorted queues): every buffer bit of 4096 random pixels
+    equals the fp32 oracle's render of those pixels."""
+    import torch
+
+    from oracle import oracle as orc
+    from roxy_dcc_amd import renderer as R
+    from roxy_dcc_amd.extract import extract_scene
+
+    world, W, H, st = helpers.load("large")
+    spp = st["samples"]
+    assert (W, H, spp) == (1920, 1080, 16)
+    ex = extract_scene(world)
+    integ = R.Integrator(ex.scene, W, H, spp)
+    frame = R.DeviceFrame(W, H)
+    cam = world.active_camera
+    mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
+    print(f"large at {W}x{H}x{spp}: one GPU pass, oracle on 4096 pixels", flush=True)
+    prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                          world._background_color, 0, spp)
+    integ.render(frame, R.camera_struct(cam, W, H), prm, 0, spp)
+    torch.cuda.synchronize()
+    pix = np.random.default_rng(7).choice(W * H, size=4096, replace=False).astype(np.int32)
+    acc = orc.Accum(len(pix))
+    oprm = orc.params(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
+                      world._background_color, 0)
+    orc.render(helpers.flat_to_oracle(ex.flat), helpers.oracle_camera(cam, W, H), oprm, 0, spp, acc, pixels=pix)
+    np.testing.assert_array_equal(frame.color.cpu().numpy().reshape(-1, 3)[pix], acc.color)
+    np.testing.assert_array_equal(frame.normal.cpu().numpy().reshape(-1, 3)[pix], acc.normal)
+    np.testing.assert_array_equal(frame.albedo.cpu().numpy().reshape(-1, 3)[pix], acc.albedo)
+    np.testing.assert_array_equal(frame.depth.cpu().numpy().reshape(-1)[pix], acc.depth)
+
+
 def test_render_independent_of_pass_split():
     a = helpers.render_pair("cornell", width=32, height=32, samples=6, spp_per_pass=6)
     b = helpers.render_pair("cornell", width=32, height=32, samples=6, spp_per_pass=1)
