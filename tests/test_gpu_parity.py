@@ -367,26 +367,29 @@ def test_untiled_frame_pixel_major_matches_oracle(width, height, spp):
     assert res["gpu_counts"]["shadow_rays"] == res["cpu_counts"]["shadow_rays"]
 
 
-def test_headline_render_matches_oracle_on_sampled_pixels():
-    """The benchmarked workload itself, the 2M-triangle scene at 1920x1080
-    and 16 spp in one pass (the link-free tree copy, eye-relative camera
-    rays, tile masks, sorted queues): every buffer bit of 4096 random pixels
-    equals the fp32 oracle's render of those pixels."""
+@pytest.mark.parametrize("name,shape", [("large", (1920, 1080, 16)), ("mesh", (512, 512, 64)),
+                                        ("cornell", (256, 256, 16))])
+def test_benchmarked_render_matches_oracle_on_sampled_pixels(name, shape):
+    """The benchmarked workloads themselves at their full sizes, each in one
+    pass as bench.py renders them (the 2M-triangle 1080p scene on the
+    link-free tree copy, eye-relative camera rays, tile masks and sorted
+    queues): every buffer bit of 4096 random pixels equals the fp32
+    oracle's render of those pixels."""
     import torch
 
     from oracle import oracle as orc
     from roxy_dcc_amd import renderer as R
     from roxy_dcc_amd.extract import extract_scene
 
-    world, W, H, st = helpers.load("large")
+    world, W, H, st = helpers.load(name)
     spp = st["samples"]
-    assert (W, H, spp) == (1920, 1080, 16)
+    assert (W, H, spp) == shape
     ex = extract_scene(world)
     integ = R.Integrator(ex.scene, W, H, spp)
     frame = R.DeviceFrame(W, H)
     cam = world.active_camera
     mode = R.TaichiRenderer._direct_light_mode_to_id(st["direct_light_mode"])
-    print(f"large at {W}x{H}x{spp}: one GPU pass, oracle on 4096 pixels", flush=True)
+    print(f"{name} at {W}x{H}x{spp}: one GPU pass, oracle on 4096 pixels", flush=True)
     prm = R.params_struct(st["max_depth"], int(world._use_sky), mode, st["direct_light_max_depth"], 10.0,
                           world._background_color, 0, spp)
     integ.render(frame, R.camera_struct(cam, W, H), prm, 0, spp)
