@@ -152,9 +152,9 @@ struct ws_view {
     int tile_cols;
     // link-free copy of the (single) job's BVH (imp_usable; null when not
     // used): slot h of heap order (root 0, children 2h+1 and 2h+2) holds,
-    // for an interior node, its children's boxes as the node record's
-    // first three 16-byte words (16 bytes apart from the next record's),
-    // a leaf child's box marked on its x axis (imp_mark); imp_nodes in
+    // for an interior node, its children's boxes as the first three of its
+    // 64-byte record's four 16-byte words (the fourth unused), a leaf
+    // child's box marked on its x axis (imp_mark); imp_nodes in
     // world space, eye_imp minus the camera position (with the eye-relative
     // copies); for a leaf, its packed (first << 8 | count) in imp_leaf.
     // imp_records = 2^depth - 1 interior slots.
