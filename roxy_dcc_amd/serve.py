@@ -8,9 +8,10 @@ traces one wavefront pass of the active jobs together (``Batch``).  The
 pass's path budget is shared out by ``share_pass`` (shortest remaining work
 first, guarded by first-come-first-served deadlines), so the device stays
 full while small jobs finish in a pass or two and large ones take what is
-left.  Finished jobs are retired and their frames read back;
-new arrivals join at the next iteration, so their work overlaps with jobs
-already on the GPU.  A job's image is the same alone and in any mix
+left (a large job shares a pass only when enough jobs queue behind it).
+Finished jobs are retired and their frames read back while the next pass
+traces; new arrivals join at the next iteration, so their work overlaps
+with jobs already on the GPU.  A job's image is the same alone and in any mix
 (sampling is keyed by job seed, pixel, sample and bounce; accumulation is
 per job in sample order), whatever samples per pass it is given.
 
@@ -134,7 +135,7 @@ def deadline_order(work, deadline):
     return sorted(range(n), key=lambda k: (deadline[k], k))
 
 
-def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None):
+def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None, join=None):
     """Split of a pass's path budget: `demands` are (pixels, samples_left)
     per job in arrival order; returns samples per pass for each (whole
     samples, at most what the job still needs; 0 = not in this pass).
@@ -144,12 +145,18 @@ def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None):
     past its deadline), each as many samples as it can use; jobs whose
     next sample no longer fits wait for a later pass, and the first job in
     the order always advances; a job whose remaining samples overshoot the
-    budget by at most a quarter of it takes them all.  A job's work is
+    budget by at most a quarter of it takes them all.  With ``join`` (a
+    pass's fixed cost, in paths traced in the same time), a job joins a
+    pass that already holds others only while (jobs in the pass) x (the
+    paths it takes) is at most ``join`` x (it and the jobs after it in the
+    order): joining delays every job of the pass by its paths, waiting
+    delays it and the jobs behind it by one pass's fixed cost; the first
+    job that does not join ends the pass.  A job's work is
     pixels x samples left x its
     cost per path (``costs``, 1 by default); ``deadlines`` are in the same
     unit, counted from now (by default: first come first served from now,
     with one pass of slack).  Low median latency like shortest-first, a
-    tail no worse than the arrival-order loop's.
+    tail close to the arrival-order loop's.
     "fair": water-filling, smallest remaining work first, each job taking
     at most an equal share of what is left, so small jobs finish in one
     pass and the budget they leave goes to the larger ones (processor
@@ -168,6 +175,7 @@ def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None):
         stretch = 0.25
         order = deadline_order(work, fifo_deadlines(work, budget) if deadlines is None else deadlines)
         left = budget
+        joined = 0
         for rank, k in enumerate(order):
             px, need = demands[k]
             take = max(0, min(need, left // max(1, px)))
@@ -176,11 +184,20 @@ def share_pass(demands, budget, policy="deadline", deadlines=None, costs=None):
             # a job whose remainder overshoots the budget by a little finishes
             # in this pass rather than taking a whole pass for its last
             # samples (the pass grows by at most a quarter)
-            if take < need and (need - take) * px <= stretch * budget:
-                stretch -= (need - take) * px / budget
+            over = (need - take) * px if take < need else 0
+            if over and over <= stretch * budget:
                 take = need
+            # joining a pass puts off the end of every job already in it by
+            # the joiner's paths; waiting for the next pass puts off the
+            # joiner and every job after it by one pass's fixed cost (``join``
+            # paths' worth of time): join while that costs less in total
+            if join is not None and joined and take and joined * take * px > join * (n - rank):
+                break
+            if take == need and over:
+                stretch -= over / budget
             out[k] = take
             left -= take * px
+            joined += take > 0
         return out
     out = [1] * n
     if policy in ("fifo", "srpt"):
@@ -228,22 +245,31 @@ class RenderServer:
     by ``share_pass``.  The drop-in renderer's pass size (2^25 paths) keeps
     the traversal kernels at their large-pass rate, so the largest jobs are
     traced as fast as the one-request-at-a-time loop traces them, while a
-    pass stays short enough (tens of ms) that new arrivals join soon.  The default policy, shortest remaining work first
-    guarded by first-come-first-served deadlines, keeps the median latency
-    of shortest-first without letting the longest jobs finish later than
-    the arrival-order loop would.  Requests are admitted in arrival order
+    pass stays short enough (tens of ms) that new arrivals join soon.  The
+    default policy, shortest remaining work first guarded by
+    first-come-first-served deadlines, keeps the median latency of
+    shortest-first without letting the longest jobs finish much later than
+    the arrival-order loop would.  Every job of a pass finishes when the
+    pass ends, so a large job joins a pass only when enough jobs queue
+    behind it to be worth the delay it adds to the others (``join_paths``);
+    small jobs ride along.  Frames of finished jobs are copied to the host
+    while the next pass traces.  Requests are admitted in arrival order
     while job slots and the frame-memory budget allow."""
 
     def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=TARGET_PATHS_PER_PASS, frame_bytes_budget=32 << 30,
-                 policy="deadline", slack_passes=0.5):
+                 policy="deadline", slack_passes=0.1, join_paths=1 << 22):
         self.cache = cache
         self.policy = policy
+        self.join_paths = join_paths
         self.max_jobs = max_jobs
         self.pass_paths = pass_paths
         # how far past its first-come-first-served finish a job may be put
         # off for shorter ones, in passes (a pass cannot end exactly on a
         # deadline)
         self.slack_passes = slack_passes
+        # a pass's fixed cost (launch tails, read-back, set-up: 2-3 ms) as
+        # paths traced in the same time; None lets every job that fits join
+        # a pass (share_pass's ``join``)
         self.frame_bytes_budget = frame_bytes_budget
         self.waiting = collections.deque()
         self.active = []       # (Request, RenderJob)
@@ -259,6 +285,7 @@ class RenderServer:
         self._batch = None
         self._batch_key = None
         self._batch_rays = {}
+        self._copy_stream = None
         self.pass_s = 0.0      # wall time inside run + read-back of the passes
 
     def _close_batch(self):
@@ -308,19 +335,25 @@ class RenderServer:
 
         self.waiting.extend(sorted(requests, key=lambda r: r.arrival))
         t0 = clock()
-        while self.waiting or self.active:
+        # finished jobs whose frames are still on the device: they are read
+        # back on a side stream while the next pass traces (a fresh host copy
+        # of a 1080p frame costs about 6 ms, mostly first-touch page faults)
+        pending = []
+        while self.waiting or self.active or pending:
             now = clock() - t0
             self._note_arrivals(now)
             self._admit(now)
             if not self.active:
-                time.sleep(idle_sleep)
+                pending = self._read_back(pending, clock, t0)
+                if self.waiting:
+                    time.sleep(idle_sleep)
                 continue
             jobs = [j for _, j in self.active]
             costs = [self._cost(r) for r, _ in self.active]
             slack = self.slack_passes * self.pass_paths * sum(costs) / len(costs)
             deadlines = [self.deadline[r.rid] - self.rays_done + slack for r, _ in self.active]
             spp = share_pass([(j.width * j.height, j.samples - j.first_sample) for j in jobs], self.pass_paths,
-                             self.policy, deadlines=deadlines, costs=costs)
+                             self.policy, deadlines=deadlines, costs=costs, join=self.join_paths)
             for j, k in zip(jobs, spp):
                 j.samples_per_pass = k
             traced = [(r, j) for (r, j), k in zip(self.active, spp) if k > 0]
@@ -335,11 +368,12 @@ class RenderServer:
             batch = self._batch
             tr = clock()
             batch.run(1)
+            pending = self._read_back(pending, clock, t0)
             recs = batch.read_jobs()
             self.pass_s += clock() - tr
             self.passes += 1
             still = [(r, j) for (r, j), k in zip(self.active, spp) if k == 0]
-            finished = False
+            finished = []
             for (req, job), rec in zip(traced, recs):
                 rays = int(rec.rays) - self._batch_rays.get(req.rid, 0)
                 self._batch_rays[req.rid] = int(rec.rays)
@@ -353,18 +387,31 @@ class RenderServer:
                 req.rays += rays
                 req.samples_done = int(rec.first_sample)
                 if rec.samples_left <= 0:
-                    finished = True
-                    torch.cuda.synchronize()
-                    req.frame = job.frame.color.cpu().numpy()
-                    req.finished = clock() - t0
-                    self.done.append(req)
+                    finished.append((req, job))
                 else:
                     still.append((req, job))
             if finished:
                 self._close_batch()
+                pending = finished
             self.active = still
         self._close_batch()
         return self.done
+
+    def _read_back(self, finished, clock, t0):
+        """Copies the frames of finished jobs to the host, on a stream of its
+        own (their passes have completed: ``read_jobs`` waited for them), and
+        marks the requests done; returns an empty list."""
+        import torch
+
+        if finished:
+            if self._copy_stream is None:
+                self._copy_stream = torch.cuda.Stream()
+            with torch.cuda.stream(self._copy_stream):
+                for req, job in finished:
+                    req.frame = job.frame.color.cpu().numpy()
+                    req.finished = clock() - t0
+                    self.done.append(req)
+        return []
 
 
 def serial_loop(cache: SceneCache, requests, paths_per_pass=TARGET_PATHS_PER_PASS, clock=time.perf_counter):

@@ -485,6 +485,34 @@ def test_share_pass_deadline_guard():
     assert share_pass([(1920 * 1080, 8)], 1 << 20) == [1]
 
 
+def test_share_pass_join_rule():
+    """With ``join`` (a pass's fixed cost, in paths), a job joins a pass that
+    already holds jobs only while (jobs in the pass) x (its paths) is at most
+    ``join`` x (it and the jobs after it in the order): small jobs ride
+    along, a large one waits for its own pass unless many jobs queue behind
+    it; the order and the samples of the jobs that join are unchanged."""
+    from roxy_dcc_amd.serve import share_pass
+
+    far = [1e12] * 3
+    three = [(960 * 540, 8), (960 * 540, 32), (512 * 512, 64)]
+    assert share_pass(three, 1 << 25, deadlines=far) == [8, 32, 64]
+    # 16.6M paths after a 4.1M-path job, two jobs from the end: 16.6M > 2 x 4.2M
+    assert share_pass(three, 1 << 25, deadlines=far, join=1 << 22) == [8, 0, 0]
+    assert share_pass(three, 1 << 25, deadlines=far, join=1 << 23) == [8, 32, 0]
+    # small jobs join a 1080p job that its deadline puts first
+    mixed = [(1920 * 1080, 16), (256 * 256, 4), (128 * 128, 16)]
+    dl = [1920 * 1080 * 16, 1e12, 1e12]
+    assert share_pass(mixed, 1 << 25, deadlines=dl) == [16, 4, 16]
+    assert share_pass(mixed, 1 << 25, deadlines=dl, join=1 << 22) == [16, 4, 16]
+    # a queue of ten equal 2M-path jobs: the k-th (from 0) joins while
+    # k x 2M <= 4.2M x (10 - k), so seven share the pass
+    ten = [(1024 * 1024, 2)] * 10
+    assert share_pass(ten, 1 << 25, deadlines=[1e12] * 10) == [2] * 10
+    assert share_pass(ten, 1 << 25, deadlines=[1e12] * 10, join=1 << 22) == [2] * 7 + [0] * 3
+    # the first job always advances, whatever its size
+    assert share_pass([(1920 * 1080, 64)], 1 << 25, join=1) == [16]
+
+
 def _random_indexed_mesh(rng, n_pos=200, n_tris=500, normals=True, uvs=True):
     from roxy_dcc_amd.world import IndexedMesh
 
