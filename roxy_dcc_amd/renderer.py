@@ -198,11 +198,20 @@ class TaichiRenderer:
         return linear_to_display(x)
 
     def _copy_display_pixels(self, apply_denoise=False):
+        import torch
+
         if apply_denoise and self._denoise:
             out = self._denoised_display()
         else:
             out = self.frame.display
-        self._image.pixels[:] = out.cpu().numpy()
+        px = self._image.pixels
+        if (isinstance(px, np.ndarray) and px.dtype == np.float32 and px.flags.c_contiguous
+                and px.shape == tuple(out.shape) and out.is_contiguous()):
+            # straight into the image's own buffer: a fresh host array for a
+            # 1080p frame costs about 4.5 ms of first-touch page faults
+            torch.from_numpy(px).copy_(out)
+        else:
+            px[:] = out.cpu().numpy()
 
     def render(self):
         import torch

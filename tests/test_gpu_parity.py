@@ -566,6 +566,13 @@ def test_taichi_renderer_end_to_end():
     s = r.last_stats
     assert s.samples_rendered == 8 and s.bvh_triangles == 36 and s.bvh_nodes == 15
     assert r.last_ray_count == s.rays_cast > 64 * 64 * 8
+    # the image holds the device's display buffer bit for bit, denoised or not
+    assert np.array_equal(img.pixels, r._denoised_display().cpu().numpy())
+    plain = Image(48, 32)
+    r = TaichiRenderer(world, plain, None, samples=4, max_depth=5, denoise=False, verbose=False)
+    r.render()
+    assert np.array_equal(plain.pixels, r.frame.display.cpu().numpy())
+    assert plain.pixels.max() > 0.0
 
 
 @pytest.mark.parametrize("interval", [1, 3])
