@@ -485,6 +485,65 @@ def test_share_pass_deadline_guard():
     assert share_pass([(1920 * 1080, 8)], 1 << 20) == [1]
 
 
+def test_server_loop_serves_every_request_once(monkeypatch):
+    """RenderServer.run's bookkeeping with a stand-in for the device batch
+    (host arrays, one pass = every job advances by its share) and a clock
+    that advances 1 ms per reading: every request finishes exactly once,
+    with all its samples and its own frame, across idle gaps between
+    arrivals; frames of the last pass are read back after the loop drains."""
+    import types
+    import contextlib
+    import torch
+    from roxy_dcc_amd import serve
+
+    class Job:
+        def __init__(self, req):
+            self.width, self.height, self.samples = req.width, req.height, req.samples
+            self.first_sample, self.samples_per_pass, self.tag = 0, 1, req
+            self.frame = types.SimpleNamespace(color=torch.full((req.height, req.width, 3), float(req.rid)))
+
+    class FakeBatch:
+        created = 0
+
+        def __init__(self, jobs, use_graph=False):
+            FakeBatch.created += 1
+            self.jobs, self.first, self.rays = jobs, [j.first_sample for j in jobs], [0] * len(jobs)
+
+        def run(self, passes):
+            for i, j in enumerate(self.jobs):
+                take = min(j.samples - self.first[i], j.samples_per_pass)
+                self.first[i] += take
+                self.rays[i] += 2 * j.width * j.height * take
+
+        def read_jobs(self):
+            return [types.SimpleNamespace(rays=r, first_sample=f, samples_left=j.samples - f)
+                    for j, f, r in zip(self.jobs, self.first, self.rays)]
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(serve, "Batch", FakeBatch)
+    monkeypatch.setattr(serve, "request_job", lambda req, scene, cam, opts, samples_per_pass: Job(req))
+    monkeypatch.setattr(torch.cuda, "Stream", lambda: None)
+    monkeypatch.setattr(torch.cuda, "stream", lambda s: contextlib.nullcontext())
+    ticks = iter(range(10 ** 6))
+    clock = lambda: next(ticks) * 1e-3
+    sizes = [(64, 64, 8), (512, 512, 64), (32, 32, 4), (1920, 1080, 16), (128, 128, 32), (16, 16, 1)]
+    arrivals = [0.0, 0.0, 0.003, 0.004, 0.5, 0.501]          # an idle gap before the last two
+    reqs = [serve.Request(i, ("s", i % 2), w, h, n, a) for i, ((w, h, n), a) in enumerate(zip(sizes, arrivals))]
+    scene = types.SimpleNamespace(hbm_bytes=lambda: 1 << 20)
+    cache = serve.SceneCache(lambda key: (scene, object(), {}), max_scenes=2)
+    server = serve.RenderServer(cache, max_jobs=4, pass_paths=1 << 20)
+    done = server.run(reqs, clock=clock, idle_sleep=0.0)
+    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    for r in done:
+        assert r.samples_done == r.samples and r.rays == 2 * r.width * r.height * r.samples
+        assert r.frame.shape == (r.height, r.width, 3) and (r.frame == r.rid).all()
+        assert r.finished >= r.arrival and r.admitted >= r.arrival
+    assert server.rays_done == sum(r.rays for r in done)
+    assert server.passes >= 16 and FakeBatch.created >= 2     # the 1080p job alone takes 16 passes of 2^20
+
+
 def test_share_pass_join_rule():
     """With ``join`` (a pass's fixed cost, in paths), a job joins a pass that
     already holds jobs only while (jobs in the pass) x (its paths) is at most
