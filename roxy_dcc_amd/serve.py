@@ -253,7 +253,8 @@ class RenderServer:
     pass ends, so a large job joins a pass only when enough jobs queue
     behind it to be worth the delay it adds to the others (``join_paths``);
     small jobs ride along.  Frames of finished jobs are copied to the host
-    while the next pass traces.  Requests are admitted in arrival order
+    while the next pass traces, into host arrays touched while their last
+    pass traced.  Requests are admitted in arrival order
     while job slots and the frame-memory budget allow."""
 
     def __init__(self, cache: SceneCache, max_jobs=32, pass_paths=TARGET_PATHS_PER_PASS, frame_bytes_budget=32 << 30,
@@ -286,6 +287,7 @@ class RenderServer:
         self._batch_key = None
         self._batch_rays = {}
         self._copy_stream = None
+        self._host = {}        # id(job) -> touched host array for its frame
         self.pass_s = 0.0      # wall time inside run + read-back of the passes
 
     def _close_batch(self):
@@ -336,8 +338,9 @@ class RenderServer:
         self.waiting.extend(sorted(requests, key=lambda r: r.arrival))
         t0 = clock()
         # finished jobs whose frames are still on the device: they are read
-        # back on a side stream while the next pass traces (a fresh host copy
-        # of a 1080p frame costs about 6 ms, mostly first-touch page faults)
+        # back on a side stream while the next pass traces (a host copy of a
+        # 1080p frame costs 1.5 ms into touched memory, about 6 ms into a
+        # fresh array)
         pending = []
         while self.waiting or self.active or pending:
             now = clock() - t0
@@ -369,6 +372,14 @@ class RenderServer:
             tr = clock()
             batch.run(1)
             pending = self._read_back(pending, clock, t0)
+            # host arrays for the frames this pass will finish, touched while
+            # it traces (a copy into touched memory is about 4x faster than
+            # into a fresh array, whose pages fault on first write)
+            for (req, job), k in zip(self.active, spp):
+                if k and k >= job.samples - job.first_sample and id(job) not in self._host:
+                    h = np.empty(tuple(job.frame.color.shape), np.float32)
+                    h[...] = 0.0
+                    self._host[id(job)] = h
             recs = batch.read_jobs()
             self.pass_s += clock() - tr
             self.passes += 1
@@ -408,7 +419,12 @@ class RenderServer:
                 self._copy_stream = torch.cuda.Stream()
             with torch.cuda.stream(self._copy_stream):
                 for req, job in finished:
-                    req.frame = job.frame.color.cpu().numpy()
+                    h = self._host.pop(id(job), None)
+                    if h is None:
+                        req.frame = job.frame.color.cpu().numpy()
+                    else:
+                        torch.from_numpy(h).copy_(job.frame.color)
+                        req.frame = h
                     req.finished = clock() - t0
                     self.done.append(req)
         return []
